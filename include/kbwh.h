@@ -1,0 +1,172 @@
+/*
+ * kbwh.h -- C ABI of the MI355X-native dense state-vector engine.
+ *
+ * The engine is a drop-in for the dense path of KBW: the Rust trait
+ * QuantumExecution (src/quantum_execution.rs:48-162) as implemented by
+ * DenseV2<F> (src/dense_v2.rs:150-342) and Block<G, S>
+ * (src/block.rs:211-381), which the C API selects by name in
+ * kbw_make_configuration (src/c_api.rs:83-120, kbw.h:48-54).  Each entry
+ * point below names the reference method it replaces.
+ *
+ * Conventions
+ *  - Every function returns int32_t: KBWH_OK (0) on success, a KBWH_ERR_*
+ *    code otherwise; kbwh_last_error() describes the last failure of the
+ *    calling thread.  No C++ exception crosses this boundary.
+ *  - Amplitudes are complex f64, interleaved {re, im}; basis index i holds
+ *    |i>, qubit q is bit q of i (dense_v2.rs:45).
+ *  - A handle owns its device memory and one HIP stream.  A handle is used by
+ *    one thread at a time (the reference's &mut self); separate handles are
+ *    independent.
+ *  - Random draws are host-drawn uniforms u in [0, 1), one per draw, so the
+ *    caller's RNG (the reference's StdRng seeded from KBW_SEED,
+ *    quantum_execution.rs:182-193) stays on the caller's side.
+ */
+#ifndef KBWH_C_ABI_H_
+#define KBWH_C_ABI_H_
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define KBWH_OK 0
+#define KBWH_ERR_EXECUTION_FAILED 1   /* KetError::ExecutionFailed: n > 32 (dense_v2.rs:155-157), Block n <= 1 (block.rs:218-220) */
+#define KBWH_ERR_INVALID_ARGUMENT 2   /* bad qubit index, null pointer, unknown gate kind */
+#define KBWH_ERR_OUT_OF_MEMORY 3      /* device allocation failed */
+#define KBWH_ERR_DEVICE 4             /* HIP runtime error (message in kbwh_last_error) */
+#define KBWH_ERR_NO_DEVICE 5          /* no HIP device visible */
+#define KBWH_ERR_ZERO_WEIGHTS 6       /* sample/measure over an all-zero distribution (the reference panics, convert.rs:60) */
+#define KBWH_ERR_CAPACITY 7           /* caller buffer too small (kbwh_dump) */
+
+/* ---- gate kinds (QuantumExecution methods, quantum_execution.rs:57-79) -- */
+#define KBWH_X 0  /* pauli_x   dense_v2.rs:166-174 */
+#define KBWH_Y 1  /* pauli_y   dense_v2.rs:176-186 */
+#define KBWH_Z 2  /* pauli_z   dense_v2.rs:188-196 */
+#define KBWH_H 3  /* hadamard  dense_v2.rs:198-209 */
+#define KBWH_P 4  /* phase     dense_v2.rs:211-221 */
+#define KBWH_RX 5 /* rx        dense_v2.rs:223-242 */
+#define KBWH_RY 6 /* ry        dense_v2.rs:244-263 */
+#define KBWH_RZ 7 /* rz        dense_v2.rs:265-277 */
+
+/* Pauli letters of a Hamiltonian term (libket Pauli, quantum_execution.rs:114-121) */
+#define KBWH_PAULI_X 1
+#define KBWH_PAULI_Y 2
+#define KBWH_PAULI_Z 3
+
+/* kbwh_new flags */
+#define KBWH_FLAG_BLOCK 1u  /* Block<_, DenseV2> semantics (block.rs): logical/physical map, LRU swaps */
+
+/* kbwh_run modes */
+#define KBWH_RUN_PER_GATE 0u /* one HBM pass per gate: the plain path */
+#define KBWH_RUN_TILED 1u    /* gates grouped into multi-gate passes over LDS tiles; same per-pair arithmetic, same result bits */
+
+/* One circuit operation (24 bytes).  `angle` is used unless `param` >= 0
+ * and a parameter vector is supplied, in which case the angle is
+ * params[param].  `ctrl_mask` has bit c set for each control qubit c. */
+typedef struct kbwh_op {
+    uint8_t kind;
+    uint8_t target;
+    uint16_t reserved;
+    int32_t param;
+    double angle;
+    uint64_t ctrl_mask;
+} kbwh_op;
+
+typedef struct kbwh_state kbwh_state;
+
+/* ---- lifetime ----------------------------------------------------------- */
+/* Replaces DenseV2::new (dense_v2.rs:151-164) and Block::new (block.rs:212-248).
+ * With KBWH_FLAG_BLOCK, block_size is the number of local qubits
+ * (KBW_BLOCK_SIZE; 0 = read KBW_BLOCK_SIZE from the environment, default 20),
+ * capped at n - 1 as in block.rs:222-228. */
+int32_t kbwh_new(uint32_t num_qubits, uint32_t flags, uint32_t block_size, kbwh_state **out);
+int32_t kbwh_free(kbwh_state *s);
+/* Replaces clear (dense_v2.rs:338-341; block.rs:378-380 also resets the map). */
+int32_t kbwh_clear(kbwh_state *s);
+int32_t kbwh_num_qubits(const kbwh_state *s, uint32_t *n);
+
+/* ---- gates ----------------------------------------------------------- */
+/* Replaces pauli_x/pauli_y/pauli_z/hadamard/phase/rx/ry/rz
+ * (quantum_execution.rs:57-79; dense_v2.rs:166-277).  A control equal to
+ * the target is dropped, as the reference's split masks do
+ * (dense_v2.rs:56-57).  `angle` is ignored by X/Y/Z/H. */
+int32_t kbwh_gate(kbwh_state *s, int32_t kind, double angle, uint32_t target,
+                  const uint32_t *control, uint32_t num_control);
+int32_t kbwh_gate_mask(kbwh_state *s, int32_t kind, double angle, uint32_t target,
+                       uint64_t ctrl_mask);
+/* A gate sequence, as QubitManager::compute_gate replays a circuit
+ * (quantum_execution.rs:310-314, 374-377).  `params` may be NULL. */
+int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                 uint32_t mode);
+
+/* ---- measurement ----------------------------------------------------- */
+/* Replaces measure_p1 (dense_v2.rs:279-294). */
+int32_t kbwh_measure_p1(kbwh_state *s, uint32_t target, double *p1);
+/* Replaces measure_collapse (dense_v2.rs:296-306). */
+int32_t kbwh_measure_collapse(kbwh_state *s, uint32_t target, int32_t result, double p);
+/* Replaces QubitManager::do_measure (quantum_execution.rs:272-292): qubits
+ * are measured in reverse list order, the k-th measured consumes
+ * uniforms[k]; bit k of *result is its outcome. */
+int32_t kbwh_measure(kbwh_state *s, const uint32_t *qubits, uint32_t num_qubits,
+                     const double *uniforms, uint64_t *result);
+
+/* ---- dump / sample / expectation -------------------------------------- */
+/* Replaces dump (dense_v2.rs:308-336; block.rs:318-376).  Entries with
+ * |a| > 1e-15 in physical index order; key bit j is qubit
+ * qubits[num_qubits-1-j].  Two calls: keys == NULL stores the count only;
+ * otherwise at most `capacity` entries are written (KBWH_ERR_CAPACITY if
+ * more exist). */
+int32_t kbwh_dump(kbwh_state *s, const uint32_t *qubits, uint32_t num_qubits, uint64_t capacity,
+                  uint64_t *count, uint64_t *keys, double *re, double *im);
+/* Replaces the default sample (quantum_execution.rs:99-102, convert.rs:54-73):
+ * one dump key per shot, drawn with uniforms[shot] under the hierarchical
+ * 64-way cumulative order (the WeightedIndex rule with a fixed association
+ * of the additions). */
+int32_t kbwh_sample(kbwh_state *s, const uint32_t *qubits, uint32_t num_qubits, uint64_t shots,
+                    const double *uniforms, uint64_t *outcomes);
+/* Replaces the default exp_value (quantum_execution.rs:109-158) with a direct
+ * <psi|P|psi> reduction per Pauli string (the state is not modified).
+ * String k is paulis/qubits[offsets[k] .. offsets[k+1]) with coefficient
+ * coeffs[k]; *out = sum_k coeffs[k] * <P_k>. */
+int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
+                       const uint32_t *offsets, const double *coeffs, uint32_t num_strings,
+                       double *out);
+
+/* ---- adjoint differentiation (no counterpart in the reference: its
+ * BatchExecution::gradient returns NativeGateUnsupported,
+ * quantum_execution.rs:420-426, and libket falls back to parameter shift) --
+ * Runs ops from |0...0> with angles params[op.param], then returns
+ * E = <psi|H|psi> and dE/dparams[j] for every j < num_params, un-applying the
+ * gates in reverse on two extra state vectors (no stored intermediates).
+ * Parameterised ops must be RX, RY, RZ or P (any controls). */
+int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops,
+                              const double *params, uint32_t num_params, const uint8_t *paulis,
+                              const uint32_t *qubits, const uint32_t *offsets,
+                              const double *coeffs, uint32_t num_strings, double *energy,
+                              double *grad);
+
+/* ---- raw access (tests, tooling) ------------------------------------- */
+/* Copies amplitudes [first, first+count) of the physical buffer. */
+int32_t kbwh_read_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, double *out);
+int32_t kbwh_write_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, const double *in);
+/* logical_to_physical[q] = physical bit of logical qubit q (identity unless Block). */
+int32_t kbwh_qubit_map(const kbwh_state *s, uint32_t *logical_to_physical);
+int32_t kbwh_swap_count(const kbwh_state *s, uint64_t *swaps);
+int32_t kbwh_synchronize(kbwh_state *s);
+/* The handle's HIP stream (hipStream_t) and device buffer, for event timing
+ * and zero-copy views. */
+int32_t kbwh_stream(const kbwh_state *s, void **stream);
+int32_t kbwh_device_buffer(const kbwh_state *s, void **ptr, uint64_t *bytes);
+
+/* ---- build info / errors --------------------------------------------- */
+/* Mirrors kbw_build_info (c_api.rs:149-163, kbw.h:64). */
+int32_t kbwh_build_info(const uint8_t **msg, uintptr_t *size);
+const char *kbwh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

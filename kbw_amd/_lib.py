@@ -1,0 +1,109 @@
+"""ctypes binding of libkbwh.so (include/kbwh.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading
+fails loudly when kbw_amd/libkbwh.so has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkbwh.so")
+
+OK = 0
+ERR_EXECUTION_FAILED = 1
+ERR_INVALID_ARGUMENT = 2
+ERR_OUT_OF_MEMORY = 3
+ERR_DEVICE = 4
+ERR_NO_DEVICE = 5
+ERR_ZERO_WEIGHTS = 6
+ERR_CAPACITY = 7
+
+X, Y, Z, H, P, RX, RY, RZ = range(8)
+GATE_NAMES = {"x": X, "y": Y, "z": Z, "h": H, "p": P, "rx": RX, "ry": RY, "rz": RZ}
+PAULI_X, PAULI_Y, PAULI_Z = 1, 2, 3
+FLAG_BLOCK = 1
+RUN_PER_GATE, RUN_TILED = 0, 1
+
+# kbwh_op, 24 bytes
+OP_DTYPE = np.dtype([("kind", "u1"), ("target", "u1"), ("reserved", "<u2"), ("param", "<i4"),
+                     ("angle", "<f8"), ("ctrl_mask", "<u8")], align=True)
+assert OP_DTYPE.itemsize == 24
+
+# Every exported symbol: (name, argtypes).  All return int32 except kbwh_last_error.
+_vp, _u32, _u64, _i32, _f64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_double
+_p = C.c_void_p
+SIGNATURES = {
+    "kbwh_new": [_u32, _u32, _u32, C.POINTER(_vp)],
+    "kbwh_free": [_vp],
+    "kbwh_clear": [_vp],
+    "kbwh_num_qubits": [_vp, C.POINTER(_u32)],
+    "kbwh_gate": [_vp, _i32, _f64, _u32, _p, _u32],
+    "kbwh_gate_mask": [_vp, _i32, _f64, _u32, _u64],
+    "kbwh_run": [_vp, _p, _u64, _p, _u32],
+    "kbwh_measure_p1": [_vp, _u32, C.POINTER(_f64)],
+    "kbwh_measure_collapse": [_vp, _u32, _i32, _f64],
+    "kbwh_measure": [_vp, _p, _u32, _p, C.POINTER(_u64)],
+    "kbwh_dump": [_vp, _p, _u32, _u64, C.POINTER(_u64), _p, _p, _p],
+    "kbwh_sample": [_vp, _p, _u32, _u64, _p, _p],
+    "kbwh_exp_value": [_vp, _p, _p, _p, _p, _u32, C.POINTER(_f64)],
+    "kbwh_adjoint_gradient": [_vp, _p, _u64, _p, _u32, _p, _p, _p, _p, _u32, C.POINTER(_f64), _p],
+    "kbwh_read_amplitudes": [_vp, _u64, _u64, _p],
+    "kbwh_write_amplitudes": [_vp, _u64, _u64, _p],
+    "kbwh_qubit_map": [_vp, _p],
+    "kbwh_swap_count": [_vp, C.POINTER(_u64)],
+    "kbwh_synchronize": [_vp],
+    "kbwh_stream": [_vp, C.POINTER(_vp)],
+    "kbwh_device_buffer": [_vp, C.POINTER(_vp), C.POINTER(_u64)],
+    "kbwh_build_info": [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)],
+    "kbwh_last_error": [],
+}
+
+
+class KbwhError(RuntimeError):
+    """A non-zero status from the C ABI."""
+
+    def __init__(self, code, message):
+        super().__init__(f"kbwh error {code}: {message}")
+        self.code = code
+
+
+_LIB = None
+
+
+def load():
+    """Load libkbwh.so once and attach the signatures."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "kbwh_last_error" else C.c_int32
+    _LIB = lib
+    return lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = load().kbwh_last_error()
+        raise KbwhError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def ptr(a):
+    """Data pointer of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def build_info():
+    lib = load()
+    msg = C.POINTER(C.c_uint8)()
+    size = C.c_size_t()
+    check(lib.kbwh_build_info(C.byref(msg), C.byref(size)))
+    return C.string_at(msg, size.value).decode()

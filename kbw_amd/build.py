@@ -1,0 +1,83 @@
+"""Builds libkbwh.so (HIP kernels for gfx950 + the C ABI) in-tree.
+
+The library is compiled with hipcc for gfx950 only.  FP contraction is off in
+every translation unit: the plain gate path reproduces the reference's
+arithmetic operand for operand, and a fused multiply-add would change the
+rounding.  Packed FP32 instructions are disabled.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libkbwh.so")
+BUILD_DIR = os.path.join(ROOT, "_build", "kbwh")
+
+HIP_SOURCES = ["kernels.hip", "state.hip", "observables.hip", "tiled.hip", "adjoint.hip"]
+CXX_SOURCES = ["coef.cpp"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
+]
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout)
+        raise RuntimeError(f"build step failed: {cmd[0]} {cmd[-1]}")
+    return out
+
+
+def build(jobs=8, verbose=False):
+    """Compile every source and link kbw_amd/libkbwh.so (always rebuilds)."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    procs, objs = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
+        objs.append(obj)
+        procs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    for src in CXX_SOURCES:
+        obj = os.path.join(BUILD_DIR, src.replace(".cpp", ".o"))
+        objs.append(obj)
+        procs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    running = []
+    for cmd in procs:
+        running.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        if len(running) >= jobs:
+            _wait(running.pop(0), verbose)
+    while running:
+        _wait(running.pop(0), verbose)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def _wait(item, verbose):
+    cmd, p = item
+    out, _ = p.communicate()
+    if p.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + out)
+        raise RuntimeError(f"compile failed: {cmd[-3]}")
+    if verbose and out.strip():
+        sys.stderr.write(out)
+
+
+def build_oracle():
+    """Compile the CPU oracle (test infrastructure) with its Makefile."""
+    subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    build_oracle()
+    print(LIB)

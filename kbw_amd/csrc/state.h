@@ -1,0 +1,63 @@
+// The state handle and the host helpers shared by the engine's translation units.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/kbwh.h"
+#include "internal.h"
+
+constexpr uint32_t kResultSlots = 4096;
+
+struct kbwh_state {
+    uint32_t n = 0;
+    bool block = false;
+    uint32_t block_size_arg = 0;
+    uint32_t nlocal = 0;  // Block: local qubits (physical bits [0, nlocal))
+    uint64_t size = 0;
+    int device = 0;
+    double2 *amp = nullptr;
+    hipStream_t stream = nullptr;
+    // reductions: per-block partials, device results, pinned host copy
+    double *partial = nullptr;
+    double *result = nullptr;
+    double *host_result = nullptr;
+    // growable device scratch (sampling levels, dump offsets, outputs)
+    double *scratch = nullptr;
+    size_t scratch_cap = 0;
+    // Block mapping (block.rs:94-98): logical <-> physical bits and LRU stamps
+    std::vector<uint32_t> l2p, p2l;
+    std::vector<uint64_t> last_access;
+    uint64_t counter = 0;
+    uint64_t swaps = 0;
+    // adjoint differentiation: two extra state vectors, allocated on first use
+    double2 *adj_phi = nullptr;
+    double2 *adj_lam = nullptr;
+    double *adj_partial = nullptr;
+    size_t adj_partial_cap = 0;
+};
+
+namespace kbwh {
+
+int32_t set_error(int32_t code, const std::string &msg);
+int32_t hip_status(hipError_t e, const char *what);
+int32_t ensure_scratch(kbwh_state *s, size_t doubles);
+int32_t local_qubit(kbwh_state *s, uint32_t logical, uint32_t *out);
+uint64_t physical_mask(const kbwh_state *s, uint64_t logical_mask);
+int32_t apply_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint64_t ctrl_mask);
+int32_t reduce_to_host(kbwh_state *s, double *out, uint32_t count);
+double uniform_scale(double total);
+int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params);
+void free_adjoint(kbwh_state *s);
+
+// Pauli strings flattened to masks: x = X|Y qubits, z = Z|Y qubits, ny = #Y.
+struct PauliMasks {
+    uint64_t x, z;
+    uint32_t ny;
+};
+int32_t pauli_masks(const kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
+                    uint32_t lo, uint32_t hi, bool physical, PauliMasks *out);
+
+}  // namespace kbwh

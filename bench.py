@@ -28,6 +28,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "30-qubit c128 gates/s + HBM TB/s; adjoint grads/s vs plain at 1e-9"
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE at 30 qubits, each
+# scaled by the same counter on a 1 GiB device copy (fetch x2.0, write x1.0).
+TRAFFIC_PAIR = 34361835520.0
+TRAFFIC_TILE = None
 
 
 def parse():
@@ -38,7 +42,7 @@ def parse():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--layers", type=int, default=20)
     ap.add_argument("--mode", choices=["per_gate", "tiled", "best"], default="best")
-    ap.add_argument("--cpu-gates", type=int, default=12, help="oracle sample size (gates); 0 = skip")
+    ap.add_argument("--cpu-gates", type=int, default=150, help="oracle sample size (gates); 0 = skip")
     ap.add_argument("--pmc", action="store_true",
                     help="counter run: a 1 GiB torch copy then 4 pair passes (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -103,10 +107,10 @@ def main():
     sim = DenseHip(n)
     stream = torch.cuda.ExternalStream(sim.stream())
     lib, h = sim._lib, sim.handle
-    mode = args.mode
-    run_mode = _lib.RUN_TILED if mode == "tiled" else _lib.RUN_PER_GATE
+    want_pg = args.mode in ("per_gate", "best")
+    want_tiled = args.mode in ("tiled", "best")
 
-    def step_events():
+    def per_gate_step():
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(nops + 1)]
         ev[0].record(stream)
         for i in range(nops):
@@ -117,51 +121,74 @@ def main():
             ev[i + 1].record(stream)
         return ev
 
+    def tiled_step():
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        sim.run(ops, mode=_lib.RUN_TILED)
+        e1.record(stream)
+        return e0, e1
+
     for _ in range(args.warmup):
-        sim.run(ops, mode=run_mode)
+        if want_tiled:
+            sim.run(ops, mode=_lib.RUN_TILED)
+        if want_pg and args.mode == "per_gate":
+            sim.run(ops, mode=_lib.RUN_PER_GATE)
     sim.synchronize()
     torch.cuda.synchronize()
 
-    t0 = time.perf_counter()
-    all_ev = []
-    for _ in range(args.steps):
-        if run_mode == _lib.RUN_PER_GATE:
-            all_ev.append(step_events())
-        else:
-            sim.run(ops, mode=run_mode)
-    sim.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    result = {"metric": METRIC, "unit": "gates/s", "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+              "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    config = {"workload": f"brickwork-{n}q-c128-{args.layers}layers-seed1", "qubits": n,
+              "layers": args.layers, "gates_per_step": nops, "state_GiB": (16 << n) / 2 ** 30,
+              "parallelism": "1 GPU"}
+    per_launch = 32.0 * (1 << n)
 
-    gates_per_s = nops * args.steps / elapsed
-    result = {
-        "metric": METRIC, "value": gates_per_s, "unit": "gates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"brickwork-{n}q-c128-{args.layers}layers-seed1", "qubits": n,
-                   "layers": args.layers, "gates_per_step": nops,
-                   "state_GiB": (16 << n) / 2 ** 30,
-                   "run_mode": "per_gate" if run_mode == _lib.RUN_PER_GATE else "tiled",
-                   "parallelism": "1 GPU"},
-        "hbm_TBps_algorithmic": alg_bytes * args.steps / elapsed / 1e12,
-    }
-    if all_ev:
-        dur = []
-        for ev in all_ev:
-            d = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(nops)])  # ms
-            dur.append(d)
-        dur = np.stack(dur)
+    pg = None
+    if want_pg:
+        pg_steps = args.steps if args.mode == "per_gate" else 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        evs = [per_gate_step() for _ in range(pg_steps)]
+        sim.synchronize()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        dur = np.stack([np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(nops)]) for ev in evs])
         pair_ms = float(dur[:, is_pair].mean())
-        per_launch = 32.0 * (1 << n)
-        achieved = per_launch / (pair_ms * 1e-3) / 1e9
-        result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                              "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                              "kernel": "k_pair (uncontrolled X/Y/H/RX/RY pass)",
-                              "launch_ms": pair_ms, "bytes_per_launch": per_launch,
-                              "launches": int(is_pair.sum()) * args.steps}
-        all_bytes = np.array([circuits.gate_bytes(n, int(o["kind"]), int(o["ctrl_mask"]), int(o["target"])) for o in ops])
-        result["per_gate_GBps_all_kernels"] = float(all_bytes.sum() / (dur.mean(axis=0).sum() * 1e-3) / 1e9)
+        ach = per_launch / (pair_ms * 1e-3) / 1e9
+        pg = {"gates_per_s": nops * pg_steps / el, "ms_per_step": 1e3 * el / pg_steps, "steps": pg_steps,
+              "hbm_TBps_algorithmic": alg_bytes * pg_steps / el / 1e12,
+              "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBPS, "traffic": TRAFFIC_PAIR,
+                           "kernel": "k_pair<uncontrolled X/Y/H/RX/RY>", "launch_ms": pair_ms,
+                           "bytes_per_launch": per_launch, "launches": int(is_pair.sum()) * pg_steps}}
+        result["per_gate"] = pg
+
+    if want_tiled:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        evs = [tiled_step() for _ in range(args.steps)]
+        sim.synchronize()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        gates, passes = sim.run_stats()
+        run_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+        launch_ms = run_ms / passes
+        ach = per_launch / (launch_ms * 1e-3) / 1e9
+        result.update({"value": nops * args.steps / el, "ms_per_step": 1e3 * el / args.steps})
+        config.update({"run_mode": "tiled", "passes_per_step": passes,
+                       "gates_per_pass": gates / passes})
+        result["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                              "frac": ach / HBM_PEAK_GBPS, "traffic": TRAFFIC_TILE,
+                              "kernel": "k_tile (12-qubit tile, gates applied in registers/LDS)",
+                              "launch_ms": launch_ms, "bytes_per_launch": per_launch,
+                              "launches": passes * args.steps}
+        result["hbm_TBps_algorithmic_per_gate_equivalent"] = alg_bytes * args.steps / el / 1e12
+    else:
+        result.update({"value": pg["gates_per_s"], "ms_per_step": pg["ms_per_step"]})
+        config["run_mode"] = "per_gate"
+        result["roofline"] = pg["roofline"]
+    result["config"] = config
 
     # expectation value of the 64-string Hamiltonian
     t1 = time.perf_counter()

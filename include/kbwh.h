@@ -155,6 +155,8 @@ int32_t kbwh_write_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, con
 /* logical_to_physical[q] = physical bit of logical qubit q (identity unless Block). */
 int32_t kbwh_qubit_map(const kbwh_state *s, uint32_t *logical_to_physical);
 int32_t kbwh_swap_count(const kbwh_state *s, uint64_t *swaps);
+/* Gates and HBM passes (kernel launches over the state) of the last kbwh_run. */
+int32_t kbwh_run_stats(const kbwh_state *s, uint64_t *gates, uint64_t *passes);
 int32_t kbwh_synchronize(kbwh_state *s);
 /* The handle's HIP stream (hipStream_t) and device buffer, for event timing
  * and zero-copy views. */

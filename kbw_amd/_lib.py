@@ -53,6 +53,7 @@ SIGNATURES = {
     "kbwh_qubit_map": [_vp, _p],
     "kbwh_swap_count": [_vp, C.POINTER(_u64)],
     "kbwh_synchronize": [_vp],
+    "kbwh_run_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64)],
     "kbwh_stream": [_vp, C.POINTER(_vp)],
     "kbwh_device_buffer": [_vp, C.POINTER(_vp), C.POINTER(_u64)],
     "kbwh_build_info": [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)],

@@ -24,7 +24,7 @@ namespace kbwh {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPPT = 4;  // pairs (or amplitudes) per thread, loads issued together
+constexpr int kPPT = 1;  // pairs (or amplitudes) per thread: one measured best on gfx950 for in-place streaming
 
 // 16-byte amplitude loads/stores with the nontemporal hint: the state is
 // streamed once per pass and is far larger than the L2 and MALL.

@@ -32,6 +32,11 @@ struct kbwh_state {
     std::vector<uint64_t> last_access;
     uint64_t counter = 0;
     uint64_t swaps = 0;
+    // tiled passes: device copy of the pass gate table
+    void *tile_gates = nullptr;
+    size_t tile_gates_cap = 0;
+    uint64_t last_passes = 0;    // HBM passes of the last kbwh_run
+    uint64_t last_gates = 0;     // gates of the last kbwh_run
     // adjoint differentiation: two extra state vectors, allocated on first use
     double2 *adj_phi = nullptr;
     double2 *adj_lam = nullptr;

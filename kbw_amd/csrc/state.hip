@@ -226,6 +226,7 @@ int32_t kbwh_free(kbwh_state *s) {
     if (s->partial) (void)hipFree(s->partial);
     if (s->result) (void)hipFree(s->result);
     if (s->scratch) (void)hipFree(s->scratch);
+    if (s->tile_gates) (void)hipFree(s->tile_gates);
     if (s->host_result) (void)hipHostFree(s->host_result);
     free_adjoint(s);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -270,6 +271,8 @@ int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const doub
     if (!s || (num_ops && !ops)) return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
     if (mode == KBWH_RUN_TILED) return run_tiled(s, ops, num_ops, params);
     if (mode != KBWH_RUN_PER_GATE) return fail(KBWH_ERR_INVALID_ARGUMENT, "unknown run mode");
+    s->last_passes = num_ops;
+    s->last_gates = num_ops;
     for (uint64_t i = 0; i < num_ops; ++i) {
         const double angle = (params && ops[i].param >= 0) ? params[ops[i].param] : ops[i].angle;
         int32_t rc = apply_gate(s, ops[i].kind, angle, ops[i].target, ops[i].ctrl_mask);
@@ -345,6 +348,13 @@ int32_t kbwh_qubit_map(const kbwh_state *s, uint32_t *l2p) {
 int32_t kbwh_swap_count(const kbwh_state *s, uint64_t *swaps) {
     if (!s || !swaps) return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
     *swaps = s->swaps;
+    return KBWH_OK;
+}
+
+int32_t kbwh_run_stats(const kbwh_state *s, uint64_t *gates, uint64_t *passes) {
+    if (!s || !gates || !passes) return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    *gates = s->last_gates;
+    *passes = s->last_passes;
     return KBWH_OK;
 }
 

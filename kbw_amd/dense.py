@@ -240,6 +240,12 @@ class DenseHip:
         check(self._lib.kbwh_swap_count(self._h, C.byref(out)))
         return out.value
 
+    def run_stats(self):
+        """(gates, HBM passes) of the last run()."""
+        g, p = C.c_uint64(), C.c_uint64()
+        check(self._lib.kbwh_run_stats(self._h, C.byref(g), C.byref(p)))
+        return g.value, p.value
+
     def synchronize(self):
         check(self._lib.kbwh_synchronize(self._h))
 

@@ -1,20 +1,298 @@
-// Adjoint differentiation (kbwh_adjoint_gradient).
+// Adjoint differentiation of E(theta) = <psi|H|psi>, psi = U_K ... U_1 |0>.
+//
+// Two state vectors besides the handle's own: phi (the circuit state, run
+// forward and then un-applied gate by gate) and lambda = H phi (un-applied in
+// step).  For a rotation R_G(theta) = exp(-i theta G / 2) on target t with
+// controls C, dU/dtheta = (-i/2) (Pi_C (x) G) U, so before un-applying gate k
+//     dE/dtheta_k = 2 Re <lambda| dU_k |phi_{k-1}> = Im <lambda| Pi_C G |phi_k>,
+// and for the phase gate P(l) = diag(1, e^{il}): dE/dl = -2 Im <lambda| Pi_{C,t=1} |phi_k>.
+// One fused pass per parameterised gate reads the four amplitudes of each
+// pair (phi and lambda), adds its term to a deterministic block reduction and
+// writes both pairs back un-applied (U_k^dagger = the same gate at -theta).
+// No intermediate state is stored, so the memory is three vectors at most.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
 #include "state.h"
 
+#pragma clang fp contract(off)
+
 namespace kbwh {
+
+namespace {
+
+constexpr int kB = 256;
+
+__device__ __forceinline__ uint64_t expand(uint64_t k, const Insert &ins) {
+    for (uint32_t j = 0; j < ins.m; ++j) {
+        const uint64_t lo = k & ins.mlo[j];
+        k = ((k ^ lo) << 1) | lo;
+    }
+    return k;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
+}
+
+// Im(conj(a) * b)
+__device__ __forceinline__ double im_conj_mul(double2 a, double2 b) { return a.x * b.y - a.y * b.x; }
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double ws[kB / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kB / 64; ++w) t += ws[w];
+    return t;
+}
+
+enum { KX = 0, KY = 1, KZ = 2, KH = 3, KP = 4, KRX = 5, KRY = 6, KRZ = 7 };
+
+// lambda (+)= coeff * P phi:  (P phi)_i = s(i^x) phi_{i^x}, s(j) = i^ny (-1)^popcount(j & z)
+__global__ __launch_bounds__(kB) void k_pauli_apply(const double2 *__restrict__ phi,
+                                                    double2 *__restrict__ lam, uint64_t size,
+                                                    uint64_t x, uint64_t z, uint32_t ny,
+                                                    double coeff, int assign) {
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= size) return;
+    const uint64_t j = i ^ x;
+    const double2 f = phi[j];
+    const uint32_t q = (ny + 2u * (uint32_t)(__popcll(j & z) & 1)) & 3u;
+    double2 v;  // f * i^q
+    if (q == 0) v = f;
+    else if (q == 1) v = make_double2(-f.y, f.x);
+    else if (q == 2) v = make_double2(-f.x, -f.y);
+    else v = make_double2(f.y, -f.x);
+    v.x *= coeff;
+    v.y *= coeff;
+    if (assign) {
+        lam[i] = v;
+    } else {
+        double2 l = lam[i];
+        l.x += v.x;
+        l.y += v.y;
+        lam[i] = l;
+    }
+}
+
+// Re <phi|lam>
+__global__ __launch_bounds__(kB) void k_dot_re(const double2 *__restrict__ a,
+                                               const double2 *__restrict__ b, uint64_t size,
+                                               double *__restrict__ partial) {
+    const uint64_t T = (uint64_t)gridDim.x * kB;
+    double acc = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x; i < size; i += T) {
+        const double2 x = a[i], y = b[i];
+        acc += x.x * y.x + x.y * y.y;
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kB) void k_sum_into(const double *__restrict__ partial, uint32_t m,
+                                                 double *__restrict__ out) {
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < m; i += kB) acc += partial[i];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// The gradient term of one parameterised gate, then both pairs un-applied.
+// (c, s) / (ph0, ph1) are the coefficients of the inverse gate.
+template <int KIND>
+__global__ __launch_bounds__(kB) void k_adj_param(double2 *__restrict__ phi, double2 *__restrict__ lam,
+                                                  uint64_t npairs, uint64_t tbit, uint64_t set,
+                                                  Insert ins, double c, double s, double2 ph0,
+                                                  double2 ph1, double *__restrict__ partial) {
+    const uint64_t T = (uint64_t)gridDim.x * kB;
+    double acc = 0.0;
+    for (uint64_t k = (uint64_t)blockIdx.x * kB + threadIdx.x; k < npairs; k += T) {
+        const uint64_t i0 = expand(k, ins) | set, i1 = i0 | tbit;
+        double2 f0 = phi[i0], f1 = phi[i1], l0 = lam[i0], l1 = lam[i1];
+        // gradient term, with phi still at phi_k
+        if constexpr (KIND == KRX) {  // Im <l|X|f>
+            acc += im_conj_mul(l0, f1) + im_conj_mul(l1, f0);
+        } else if constexpr (KIND == KRY) {  // Im <l|Y|f>: (Yf)_0 = -i f1, (Yf)_1 = i f0
+            acc += im_conj_mul(l0, make_double2(f1.y, -f1.x)) + im_conj_mul(l1, make_double2(-f0.y, f0.x));
+        } else if constexpr (KIND == KRZ) {  // Im <l|Z|f>
+            acc += im_conj_mul(l0, f0) - im_conj_mul(l1, f1);
+        } else {  // P: -2 Im <l|Pi_1|f>
+            acc += -2.0 * im_conj_mul(l1, f1);
+        }
+        // un-apply (the reference's closures, dense_v2.rs:216, 229-258, 271-272)
+        if constexpr (KIND == KRX) {
+            const double2 a0 = f0, a1 = f1, b0 = l0, b1 = l1;
+            f0 = make_double2(c * a0.x + s * a1.y, c * a0.y - s * a1.x);
+            f1 = make_double2(s * a0.y + c * a1.x, -s * a0.x + c * a1.y);
+            l0 = make_double2(c * b0.x + s * b1.y, c * b0.y - s * b1.x);
+            l1 = make_double2(s * b0.y + c * b1.x, -s * b0.x + c * b1.y);
+        } else if constexpr (KIND == KRY) {
+            const double2 a0 = f0, a1 = f1, b0 = l0, b1 = l1;
+            f0 = make_double2(c * a0.x - s * a1.x, c * a0.y - s * a1.y);
+            f1 = make_double2(s * a0.x + c * a1.x, s * a0.y + c * a1.y);
+            l0 = make_double2(c * b0.x - s * b1.x, c * b0.y - s * b1.y);
+            l1 = make_double2(s * b0.x + c * b1.x, s * b0.y + c * b1.y);
+        } else if constexpr (KIND == KRZ) {
+            f0 = cmul(f0, ph0);
+            f1 = cmul(f1, ph1);
+            l0 = cmul(l0, ph0);
+            l1 = cmul(l1, ph1);
+        } else {
+            f1 = cmul(f1, ph1);
+            l1 = cmul(l1, ph1);
+        }
+        phi[i0] = f0;
+        phi[i1] = f1;
+        lam[i0] = l0;
+        lam[i1] = l1;
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+inline uint32_t grid_for(uint64_t items, uint64_t per) { return (uint32_t)((items + per - 1) / per); }
+
+int32_t ensure_vec(double2 **p, uint64_t size) {
+    if (*p) return KBWH_OK;
+    return hip_status(hipMalloc(p, size * sizeof(double2)), "adjoint vector");
+}
+
+}  // namespace
+
 void free_adjoint(kbwh_state *s) {
     if (s->adj_phi) (void)hipFree(s->adj_phi);
     if (s->adj_lam) (void)hipFree(s->adj_lam);
     if (s->adj_partial) (void)hipFree(s->adj_partial);
     s->adj_phi = s->adj_lam = nullptr;
     s->adj_partial = nullptr;
+    s->adj_partial_cap = 0;
 }
+
 }  // namespace kbwh
+
+using namespace kbwh;
 
 extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops,
                                          const double *params, uint32_t num_params,
                                          const uint8_t *paulis, const uint32_t *qubits,
                                          const uint32_t *offsets, const double *coeffs,
                                          uint32_t num_strings, double *energy, double *grad) {
-    return kbwh::set_error(KBWH_ERR_INVALID_ARGUMENT, "adjoint not built");
+    if (!s || !energy || (num_ops && !ops) || (num_params && (!params || !grad)) ||
+        (num_strings && (!offsets || !coeffs)))
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->block) return set_error(KBWH_ERR_INVALID_ARGUMENT, "adjoint gradient needs a flat state");
+    const uint32_t n = s->n;
+    const uint64_t size = s->size;
+    for (uint64_t i = 0; i < num_ops; ++i) {
+        const kbwh_op &op = ops[i];
+        if (op.kind > KRZ || op.target >= n || (n < 64 && (op.ctrl_mask >> n)))
+            return set_error(KBWH_ERR_INVALID_ARGUMENT, "bad op");
+        if (op.param >= 0) {
+            if ((uint32_t)op.param >= num_params)
+                return set_error(KBWH_ERR_INVALID_ARGUMENT, "param index out of range");
+            if (!(op.kind == KRX || op.kind == KRY || op.kind == KRZ || op.kind == KP))
+                return set_error(KBWH_ERR_INVALID_ARGUMENT, "parameterised ops must be RX, RY, RZ or P");
+        }
+    }
+    std::vector<PauliMasks> masks(num_strings);
+    for (uint32_t k = 0; k < num_strings; ++k) {
+        int32_t rc = pauli_masks(s, paulis, qubits, offsets[k], offsets[k + 1], false, &masks[k]);
+        if (rc) return rc;
+    }
+    int32_t rc = ensure_vec(&s->adj_phi, size);
+    if (rc == KBWH_OK) rc = ensure_vec(&s->adj_lam, size);
+    if (rc) return rc;
+    uint64_t nparam_ops = 0;
+    for (uint64_t i = 0; i < num_ops; ++i) nparam_ops += ops[i].param >= 0;
+    const uint32_t nb = reduce_blocks(size);  // the largest reduction grid below (k_dot_re over all amplitudes)
+    const size_t need = (size_t)nb + nparam_ops + 2;
+    if (s->adj_partial_cap < need) {
+        if (s->adj_partial) {
+            hipError_t e = hipStreamSynchronize(s->stream);
+            if (e == hipSuccess) e = hipFree(s->adj_partial);
+            if (e != hipSuccess) return hip_status(e, "adjoint scratch");
+        }
+        s->adj_partial = nullptr;
+        s->adj_partial_cap = 0;
+        rc = hip_status(hipMalloc(&s->adj_partial, need * sizeof(double)), "adjoint scratch");
+        if (rc) return rc;
+        s->adj_partial_cap = need;
+    }
+    double *partial = s->adj_partial;
+    double *slots = partial + nb;  // [0] = energy, [1 + j] = term of the j-th parameterised op
+    hipStream_t st = s->stream;
+
+    // forward: phi = U |0>, with tiled passes on the phi buffer
+    hipError_t e = launch_init(s->adj_phi, size, st);
+    if (e != hipSuccess) return hip_status(e, "adjoint init");
+    rc = run_tiled_on(s, s->adj_phi, ops, num_ops, params);
+    if (rc) return rc;
+    // lambda = H phi and E = Re <phi|lambda>
+    const uint32_t g1 = grid_for(size, kB);
+    if (num_strings == 0) {
+        e = launch_init(s->adj_lam, size, st);  // H = 0: lambda = 0 (then overwritten below)
+        if (e == hipSuccess) {
+            k_pauli_apply<<<g1, kB, 0, st>>>(s->adj_phi, s->adj_lam, size, 0, 0, 0, 0.0, 1);
+            e = hipGetLastError();
+        }
+    }
+    for (uint32_t k = 0; k < num_strings && e == hipSuccess; ++k) {
+        k_pauli_apply<<<g1, kB, 0, st>>>(s->adj_phi, s->adj_lam, size, masks[k].x, masks[k].z,
+                                         masks[k].ny, coeffs[k], k == 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        const uint32_t gd = reduce_blocks(size);
+        k_dot_re<<<gd, kB, 0, st>>>(s->adj_phi, s->adj_lam, size, partial);
+        k_sum_into<<<1, kB, 0, st>>>(partial, gd, slots);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return hip_status(e, "adjoint lambda");
+    // reverse sweep
+    uint64_t slot = nparam_ops;
+    for (uint64_t ii = num_ops; ii-- > 0 && e == hipSuccess;) {
+        const kbwh_op &op = ops[ii];
+        const double angle = (op.param >= 0) ? params[op.param] : op.angle;
+        const uint64_t tbit = 1ull << op.target;
+        const uint64_t m = op.ctrl_mask & ~tbit;
+        if (op.param < 0) {
+            // a fixed gate: un-apply it on both vectors
+            const double inv = (op.kind == KRX || op.kind == KRY || op.kind == KRZ || op.kind == KP) ? -angle : angle;
+            e = launch_gate(s->adj_phi, n, op.kind, inv, op.target, m, st);
+            if (e == hipSuccess) e = launch_gate(s->adj_lam, n, op.kind, inv, op.target, m, st);
+            continue;
+        }
+        --slot;
+        const Coef k = make_coef(op.kind, -angle);
+        const uint32_t c = (uint32_t)__builtin_popcountll(m);
+        const uint64_t npairs = 1ull << (n - 1 - c);
+        const Insert ins = make_insert(m | tbit);
+        const uint32_t gb = reduce_blocks(npairs);
+        switch (op.kind) {
+        case KRX: k_adj_param<KRX><<<gb, kB, 0, st>>>(s->adj_phi, s->adj_lam, npairs, tbit, m, ins, k.c, k.s, k.ph0, k.ph1, partial); break;
+        case KRY: k_adj_param<KRY><<<gb, kB, 0, st>>>(s->adj_phi, s->adj_lam, npairs, tbit, m, ins, k.c, k.s, k.ph0, k.ph1, partial); break;
+        case KRZ: k_adj_param<KRZ><<<gb, kB, 0, st>>>(s->adj_phi, s->adj_lam, npairs, tbit, m, ins, k.c, k.s, k.ph0, k.ph1, partial); break;
+        default: k_adj_param<KP><<<gb, kB, 0, st>>>(s->adj_phi, s->adj_lam, npairs, tbit, m, ins, k.c, k.s, k.ph0, k.ph1, partial); break;
+        }
+        k_sum_into<<<1, kB, 0, st>>>(partial, gb, slots + 1 + slot);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return hip_status(e, "adjoint reverse");
+    std::vector<double> host(nparam_ops + 1);
+    e = hipMemcpyAsync(host.data(), slots, host.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_status(e, "adjoint read-back");
+    *energy = host[0];
+    for (uint32_t j = 0; j < num_params; ++j) grad[j] = 0.0;
+    uint64_t j = 0;
+    for (uint64_t i = 0; i < num_ops; ++i)
+        if (ops[i].param >= 0) grad[ops[i].param] += host[1 + j++];
+    return KBWH_OK;
 }

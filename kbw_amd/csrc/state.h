@@ -55,6 +55,8 @@ int32_t apply_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint6
 int32_t reduce_to_host(kbwh_state *s, double *out, uint32_t count);
 double uniform_scale(double total);
 int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params);
+int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
+                     const double *params);
 void free_adjoint(kbwh_state *s);
 
 // Pauli strings flattened to masks: x = X|Y qubits, z = Z|Y qubits, ny = #Y.

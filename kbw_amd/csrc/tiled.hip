@@ -291,10 +291,23 @@ void assign_positions(uint32_t n, const Pass &p, int pos_of[64], uint32_t q_at[T
 }  // namespace
 
 int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params) {
+    return run_tiled_on(s, s->amp, ops, num_ops, params);
+}
+
+int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
+                     const double *params) {
     if (s->n < TL) {
         // the whole state is smaller than one tile: one pass per gate
-        return kbwh_run(s, ops, num_ops, params, KBWH_RUN_PER_GATE);
+        if (amp == s->amp) return kbwh_run(s, ops, num_ops, params, KBWH_RUN_PER_GATE);
+        for (uint64_t i = 0; i < num_ops; ++i) {
+            const double angle = (params && ops[i].param >= 0) ? params[ops[i].param] : ops[i].angle;
+            hipError_t e = launch_gate(amp, s->n, ops[i].kind, angle, ops[i].target,
+                                       ops[i].ctrl_mask & ~(1ull << ops[i].target), s->stream);
+            if (e != hipSuccess) return hip_status(e, "gate launch");
+        }
+        return KBWH_OK;
     }
+    if (s->block && amp != s->amp) return set_error(KBWH_ERR_INVALID_ARGUMENT, "Block state on a side buffer");
     const uint32_t n = s->n;
     s->last_passes = 0;
     for (uint64_t i = 0; i < num_ops; ++i) {
@@ -369,7 +382,7 @@ int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const dou
         const uint32_t grid = (uint32_t)(1ull << (n - TL));
         const TGate *dg = (const TGate *)s->tile_gates;
         for (const Launch &L : launches) {
-            k_tile<<<grid, TT, 0, s->stream>>>(s->amp, L.spec, dg + L.first, L.count);
+            k_tile<<<grid, TT, 0, s->stream>>>(amp, L.spec, dg + L.first, L.count);
             e = hipGetLastError();
             if (e != hipSuccess) return hip_status(e, "k_tile launch");
         }

@@ -1,0 +1,99 @@
+"""Adjoint gradients (kbwh_adjoint_gradient) against parameter shift on the
+oracle, central finite differences, the GPU plain path, and torch autograd."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from kbw_amd import DenseHip, circuits
+from kbw_amd import autograd as kag
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_energy(n, ops, params, ham):
+    o = oracle.DenseV2(n)
+    o.run(ops, params)
+    return o.exp_value(ham.pauli_strings, ham.coefficients)
+
+
+def oracle_shift(n, ops, params, ham):
+    g = np.zeros(len(params))
+    for j in range(len(params)):
+        p = params.copy()
+        p[j] += math.pi / 2
+        ep = oracle_energy(n, ops, p, ham)
+        p[j] -= math.pi
+        em = oracle_energy(n, ops, p, ham)
+        g[j] = 0.5 * (ep - em)
+    return g
+
+
+@pytest.mark.parametrize("n,layers,ham_kind", [(6, 2, "tfim"), (10, 3, "xxz"), (13, 2, "tfim")])
+def test_adjoint_matches_oracle_parameter_shift(n, layers, ham_kind):
+    print(f"adjoint n={n}", flush=True)
+    ops, params = circuits.hea(n, layers, seed=n)
+    ham = circuits.tfim(n, seed=1) if ham_kind == "tfim" else circuits.xxz(n, seed=2)
+    sim = DenseHip(n)
+    e, g = sim.adjoint_gradient(ops, params, ham)
+    assert e == pytest.approx(oracle_energy(n, ops, params, ham), abs=1e-12)
+    ref = oracle_shift(n, ops, params, ham)
+    assert np.max(np.abs(g - ref)) < 1e-10
+    # the handle's own state is untouched
+    st = sim.state()
+    assert st[0] == 1 and np.count_nonzero(st) == 1
+
+
+def test_adjoint_controlled_and_phase_vs_finite_differences():
+    n = 9
+    rng = np.random.default_rng(4)
+    ops = circuits.random_circuit(rng, n, 60, max_controls=2)
+    kinds = ops["kind"]
+    par = np.isin(kinds, [4, 5, 6, 7])
+    ops["param"][par] = np.arange(par.sum())
+    params = ops["angle"][par].copy()
+    ham = circuits.random_hamiltonian(rng, n, 10)
+    sim = DenseHip(n)
+    _, g = sim.adjoint_gradient(ops, params, ham)
+    h = 1e-5
+    fd = np.zeros_like(params)
+    for j in range(len(params)):
+        p = params.copy()
+        p[j] += h
+        ep = oracle_energy(n, ops, p, ham)
+        p[j] -= 2 * h
+        em = oracle_energy(n, ops, p, ham)
+        fd[j] = (ep - em) / (2 * h)
+    assert np.max(np.abs(g - fd)) < 1e-7
+
+
+def test_adjoint_24q_subset_vs_plain_parameter_shift():
+    """The 24-qubit, 12-layer, 576-parameter ansatz with the XXZ chain: the
+    adjoint gradient against the GPU plain path's shift rule on 6 parameters."""
+    n = 24
+    ops, params = circuits.hea(n, 12, seed=7)
+    ham = circuits.xxz(n, seed=8)
+    sim = DenseHip(n)
+    e, g = sim.adjoint_gradient(ops, params, ham)
+    assert len(g) == 576
+    idx = [0, 100, 287, 300, 451, 575]
+    ps = kag.parameter_shift(sim, ops, params, ham, indices=idx)
+    assert np.max(np.abs(g[idx] - ps)) < 1e-9
+    e2 = kag.energy(sim, ops, params, ham)
+    assert e == pytest.approx(e2, abs=1e-11)
+
+
+def test_torch_autograd_function():
+    import torch
+
+    n = 8
+    ops, params = circuits.hea(n, 2, seed=3)
+    ham = circuits.tfim(n, seed=3)
+    sim = DenseHip(n)
+    p = torch.tensor(params, dtype=torch.float64, requires_grad=True)
+    e = kag.expval(p, sim, ops, ham)
+    (2.0 * e).backward()
+    _, g = sim.adjoint_gradient(ops, params, ham)
+    assert np.allclose(p.grad.numpy(), 2.0 * g, rtol=0, atol=1e-15)
+    assert float(e) == pytest.approx(oracle_energy(n, ops, params, ham), abs=1e-12)

@@ -1,21 +1,23 @@
 // Multi-gate tiled passes (KBWH_RUN_TILED).
 //
 // A pass applies a run of consecutive gates to the whole state in one HBM
-// read and one HBM write.  The state is cut into tiles of 2^12 amplitudes:
-// a tile is the set of indices that vary over 12 chosen qubits Q (the tile
-// positions) with every other bit fixed by the tile number.  Q always holds
-// qubits 0..2, so every 128-byte line of the state belongs to one tile and
-// is read whole.  A workgroup of 256 threads holds one tile in registers, 16
-// amplitudes per thread:
-//   tile position 0..5   -> lane bits      (exchange with cross-lane shuffles)
-//   tile position 6..7   -> wave bits      (exchange through LDS)
-//   tile position 8..11  -> register index (no exchange)
+// read and one HBM write.  The state is cut into tiles of 2^TL amplitudes
+// (TL = 11 by default): a tile is the set of indices that vary over TL chosen
+// qubits Q (the tile positions) with every other bit fixed by the tile
+// number.  Q always holds qubits 0..2, so every 128-byte line of the state
+// belongs to one tile and is read whole.  A workgroup of 256 threads holds
+// one tile in registers, 2^(TL-8) amplitudes per thread:
+//   tile position 0..5    -> lane bits      (exchange with cross-lane shuffles)
+//   tile position 6..7    -> wave bits      (exchange through LDS)
+//   tile position 8..TL-1 -> register index (no exchange)
 // A non-diagonal gate (X, Y, H, RX, RY) needs its target in Q; diagonal gates
 // (Z, P, RZ) and every control may sit anywhere (bits outside Q are constant
-// over a tile).  Gates are applied in program order with the reference's
-// per-pair arithmetic (dense_v2.rs:166-277), so a tiled run produces the same
-// amplitude bits as one pass per gate; only the number of HBM passes changes.
+// over a tile).  Gates are applied in program order and every output equals
+// the reference's per-pair closure (dense_v2.rs:166-277) in value, so a tiled
+// run produces the same amplitudes as one pass per gate; only the number of
+// HBM passes changes.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -29,10 +31,27 @@ namespace kbwh {
 
 namespace {
 
-constexpr int TL = 12;           // tile qubits
-constexpr int TT = 256;          // threads per tile
-constexpr int TR = 16;           // amplitudes per thread
+// Tile shape, measured on MI355X for the 30-qubit brickwork circuit: 2^11
+// amplitudes per tile (8 per thread), one tile per workgroup, 4 waves per
+// SIMD beat 2^12-amplitude tiles and an LDS-staged persistent pipeline.
+#ifndef KBWH_TILE_BITS
+#define KBWH_TILE_BITS 11
+#endif
+#ifndef KBWH_TILE_PIPELINE
+#define KBWH_TILE_PIPELINE 0
+#endif
+#ifndef KBWH_TILE_WAVES
+#define KBWH_TILE_WAVES 4
+#endif
+#ifndef KBWH_TILES_PER_CU
+#define KBWH_TILES_PER_CU 3
+#endif
+constexpr int TL = KBWH_TILE_BITS;  // tile qubits
+constexpr int TT = 256;             // threads per tile
+constexpr int TR = 1 << (TL - 8);   // amplitudes per thread (register bits = TL - 8)
 constexpr uint32_t kOutside = 255;
+constexpr int kChunk = 4;        // register slots per LDS exchange round (16 KiB of LDS)
+constexpr uint32_t kTilesPerCU = KBWH_TILES_PER_CU;  // resident workgroups per CU (persistent grid)
 
 struct TGate {
     uint32_t kind;
@@ -41,197 +60,355 @@ struct TGate {
     uint32_t pad;
     uint64_t ocm;   // controls outside the tile, as global bits
     uint64_t otb;   // the target's global bit (diagonal gates with target outside Q)
-    double c, s;
+    // pair gates: out coefficients (uniform part u, part times the half's sign s)
+    double au, as, bu, bs, cru, crs, ciu, cis, d;
+    // diagonal gates: the phases of the |0> and |1> halves
     double ph0x, ph0y, ph1x, ph1y;
+    double pad2;
 };
-static_assert(sizeof(TGate) == 80, "TGate layout");
+static_assert(sizeof(TGate) == 144, "TGate layout");
 
 struct TileSpec {
     uint64_t qbit[TL];  // qbit[b] = 1 << Q[b]
-    Insert ins;         // the positions of Q, to spread the tile number over the other bits
+    uint64_t qlo[TL];   // (1 << p) - 1 for the positions p of Q in ascending order
 };
 
 enum { KX = 0, KY = 1, KZ = 2, KH = 3, KP = 4, KRX = 5, KRY = 6, KRZ = 7 };
 
-__device__ __forceinline__ uint64_t expand(uint64_t k, const Insert &ins) {
-    for (uint32_t j = 0; j < ins.m; ++j) {
-        const uint64_t lo = k & ins.mlo[j];
-        k = ((k ^ lo) << 1) | lo;
-    }
-    return k;
+// Amplitudes in the tile kernel are clang vector values (selects on them are
+// value selects; HIP's double2 class would turn a ?: into a select between
+// addresses and push the register array out to scratch memory).
+typedef double dv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ dv2 mk2(double x, double y) {
+    dv2 r;
+    r.x = x;
+    r.y = y;
+    return r;
 }
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    double2 r;
+__device__ __forceinline__ dv2 cmul(dv2 a, dv2 b) {
+    dv2 r;
     r.x = a.x * b.x - a.y * b.y;
     r.y = a.y * b.x + a.x * b.y;
     return r;
 }
 
-// The pair closures of dense_v2.rs:166-263 (the non-diagonal gates).
-template <int KIND>
-__device__ __forceinline__ void pair_math(double2 &k0, double2 &k1, double c, double s) {
-    const double2 t0 = k0, t1 = k1;
-    if constexpr (KIND == KX) {
-        k0 = t1;
-        k1 = t0;
-    } else if constexpr (KIND == KY) {
-        k0 = cmul(t1, make_double2(-0.0, -1.0));
-        k1 = cmul(t0, make_double2(0.0, 1.0));
-    } else if constexpr (KIND == KH) {
-        const double f = 0.70710678118654752440;
-        k0.x = (t0.x + t1.x) * f;
-        k0.y = (t0.y + t1.y) * f;
-        k1.x = (t0.x - t1.x) * f;
-        k1.y = (t0.y - t1.y) * f;
-    } else if constexpr (KIND == KRX) {
-        k0.x = c * t0.x + s * t1.y;
-        k0.y = c * t0.y - s * t1.x;
-        k1.x = s * t0.y + c * t1.x;
-        k1.y = -s * t0.x + c * t1.y;
-    } else if constexpr (KIND == KRY) {
-        k0.x = c * t0.x - s * t1.x;
-        k0.y = c * t0.y - s * t1.y;
-        k1.x = s * t0.x + c * t1.x;
-        k1.y = s * t0.y + c * t1.y;
-    }
+__device__ __forceinline__ dv2 shfl_xor2(dv2 v, int mask) {
+    return mk2(__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64));
 }
 
-__device__ __forceinline__ double2 shfl_xor2(double2 v, int mask) {
-    double2 r;
-    r.x = __shfl_xor(v.x, mask, 64);
-    r.y = __shfl_xor(v.y, mask, 64);
-    return r;
+// Every pair closure of the reference (dense_v2.rs:166-263), written for one
+// output amplitude from its own value m and its partner's value o:
+//     out.re = ((A m.re + B o.re) + Cr o.im) * D
+//     out.im = ((A m.im + B o.im) + Ci o.re) * D
+// with coefficients that depend on the half (sigma = +1 for the |1> amplitude,
+// -1 for the |0> one):
+//     X:  A=0     B=1        Cr=0       Ci=0      D=1
+//     Y:  A=0     B=0        Cr=-sigma  Ci=sigma  D=1
+//     H:  A=-sigma B=1       Cr=0       Ci=0      D=1/sqrt2
+//     RX: A=c     B=0        Cr=s       Ci=-s     D=1
+//     RY: A=c     B=sigma*s  Cr=0       Ci=0      D=1
+// Adding a zero product, multiplying by 1 and taking a product by +-1 are
+// exact, and the order of two addends does not change a rounded sum, so each
+// output equals the reference's closure in value (the sign of a zero aside).
+struct PairCoef {
+    double A, B, Cr, Ci, D;
+};
+
+__device__ __forceinline__ PairCoef pair_coef(const TGate &G, double sigma) {
+    PairCoef k;
+    k.A = G.au + G.as * sigma;
+    k.B = G.bu + G.bs * sigma;
+    k.Cr = G.cru + G.crs * sigma;
+    k.Ci = G.ciu + G.cis * sigma;
+    k.D = G.d;
+    return k;
+}
+
+__device__ __forceinline__ dv2 pair_out(dv2 m, dv2 o, const PairCoef &k) {
+    return mk2(((k.A * m.x + k.B * o.x) + k.Cr * o.y) * k.D, ((k.A * m.y + k.B * o.y) + k.Ci * o.x) * k.D);
 }
 
 // Target on register bit RB: both amplitudes of a pair live in one thread.
-template <int KIND, int RB>
-__device__ __forceinline__ void reg_pair(double2 (&v)[TR], uint32_t tid, uint32_t cm, double c,
-                                         double s) {
+// SWAP: the X gate (dense_v2.rs:168-170), a pure exchange.  CTRL: some
+// control lies inside the tile, so only the pairs whose index matches cm change.
+template <int RB, bool SWAP, bool CTRL>
+__device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, uint32_t cm, const PairCoef &k0,
+                                         const PairCoef &k1) {
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
         if ((r >> RB) & 1) continue;
-        const uint32_t j0 = tid | ((uint32_t)r << 8);
-        if ((j0 & cm) == cm) pair_math<KIND>(v[r], v[r | (1 << RB)], c, s);
+        const int r1 = r | (1 << RB);
+        const dv2 t0 = v[r], t1 = v[r1];
+        dv2 o0, o1;
+        if constexpr (SWAP) {
+            o0 = t1;
+            o1 = t0;
+        } else {
+            o0 = pair_out(t0, t1, k0);
+            o1 = pair_out(t1, t0, k1);
+        }
+        if constexpr (CTRL) {
+            const uint32_t j0 = tid | ((uint32_t)r << 8);
+            const bool ok = (j0 & cm) == cm;
+            v[r] = ok ? o0 : t0;
+            v[r1] = ok ? o1 : t1;
+        } else {
+            v[r] = o0;
+            v[r1] = o1;
+        }
     }
 }
 
-// Target on a thread bit: each thread computes the output of its own
-// amplitude from (its value, the partner's value) with the same pair closure.
-template <int KIND>
-__device__ __forceinline__ void exchanged_pair(double2 (&v)[TR], const double2 (&p)[TR], uint32_t tid,
-                                               uint32_t tpos, uint32_t cm, double c, double s) {
-    const bool one = (tid >> tpos) & 1u;
+template <bool SWAP, bool CTRL>
+__device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, uint32_t tp,
+                                            uint32_t cm, const PairCoef &k) {
+    if (tp < 6) {
+        const int lm = 1 << tp;
 #pragma unroll
-    for (int r = 0; r < TR; ++r) {
-        const uint32_t j = tid | ((uint32_t)r << 8);
-        if ((j & cm) != cm) continue;
-        double2 k0 = one ? p[r] : v[r];
-        double2 k1 = one ? v[r] : p[r];
-        pair_math<KIND>(k0, k1, c, s);
-        v[r] = one ? k1 : k0;
-    }
-}
-
-template <int KIND>
-__device__ __forceinline__ void apply_pair(double2 (&v)[TR], double2 *lds, uint32_t tid,
-                                           const TGate &G) {
-    const uint32_t tp = G.tpos, cm = G.cm;
-    const double c = G.c, s = G.s;
-    if (tp >= 8) {
-        switch (tp - 8) {
-        case 0: reg_pair<KIND, 0>(v, tid, cm, c, s); break;
-        case 1: reg_pair<KIND, 1>(v, tid, cm, c, s); break;
-        case 2: reg_pair<KIND, 2>(v, tid, cm, c, s); break;
-        default: reg_pair<KIND, 3>(v, tid, cm, c, s); break;
+        for (int r = 0; r < TR; ++r) {
+            const dv2 o = shfl_xor2(v[r], lm);
+            const dv2 out = SWAP ? o : pair_out(v[r], o, k);
+            if constexpr (CTRL) {
+                const uint32_t j = tid | ((uint32_t)r << 8);
+                v[r] = ((j & cm) == cm) ? out : v[r];
+            } else {
+                v[r] = out;
+            }
         }
         return;
     }
-    double2 p[TR];
-    if (tp < 6) {
+    // wave bits: through LDS, kChunk register slots at a time
+    const uint32_t pt = tid ^ (1u << tp);
 #pragma unroll
-        for (int r = 0; r < TR; ++r) p[r] = shfl_xor2(v[r], 1 << tp);
-    } else {
+    for (int q = 0; q < TR; q += kChunk) {
         __syncthreads();  // earlier exchange reads are done
 #pragma unroll
-        for (int r = 0; r < TR; ++r) lds[tid | (r << 8)] = v[r];
+        for (int r = 0; r < kChunk; ++r) lds[tid | (r << 8)] = v[q + r];
         __syncthreads();
-        const uint32_t pt = tid ^ (1u << tp);
 #pragma unroll
-        for (int r = 0; r < TR; ++r) p[r] = lds[pt | (r << 8)];
-    }
-    exchanged_pair<KIND>(v, p, tid, tp, cm, c, s);
-}
-
-// Z, P and RZ: each amplitude is multiplied on its own (dense_v2.rs:190-191,
-// 215-216, 271-272).
-template <int KIND>
-__device__ __forceinline__ void apply_diag(double2 (&v)[TR], uint64_t base, uint32_t tid,
-                                           const TGate &G) {
-    const double2 ph0 = make_double2(G.ph0x, G.ph0y), ph1 = make_double2(G.ph1x, G.ph1y);
-    const bool outside = G.tpos == kOutside;
-    const bool obit = (base & G.otb) != 0;
-#pragma unroll
-    for (int r = 0; r < TR; ++r) {
-        const uint32_t j = tid | ((uint32_t)r << 8);
-        if ((j & G.cm) != G.cm) continue;
-        const bool one = outside ? obit : ((j >> G.tpos) & 1u) != 0;
-        if constexpr (KIND == KZ) {
-            if (one) v[r] = cmul(v[r], make_double2(-1.0, -0.0));
-        } else if constexpr (KIND == KP) {
-            if (one) v[r] = cmul(v[r], ph1);
-        } else {
-            v[r] = cmul(v[r], one ? ph1 : ph0);
+        for (int r = 0; r < kChunk; ++r) {
+            const dv2 o = lds[pt | (r << 8)];
+            const dv2 out = SWAP ? o : pair_out(v[q + r], o, k);
+            if constexpr (CTRL) {
+                const uint32_t j = tid | ((uint32_t)(q + r) << 8);
+                v[q + r] = ((j & cm) == cm) ? out : v[q + r];
+            } else {
+                v[q + r] = out;
+            }
         }
     }
 }
 
-__global__ __launch_bounds__(TT) void k_tile(double2 *__restrict__ a, TileSpec ts,
-                                             const TGate *__restrict__ gates, uint32_t ng) {
-    __shared__ double2 lds[TT * TR];
-    const uint32_t tid = threadIdx.x;
-    const uint64_t base = expand(blockIdx.x, ts.ins);
-    uint64_t dt = base;
-#pragma unroll
-    for (int b = 0; b < 8; ++b)
-        if ((tid >> b) & 1u) dt |= ts.qbit[b];
-    double2 v[TR];
-    typedef double d2v __attribute__((ext_vector_type(2)));
+template <bool SWAP, bool CTRL>
+__device__ __forceinline__ void apply_pair_t(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
+    const uint32_t tp = G.tpos, cm = G.cm;
+    if (tp >= 8) {
+        const PairCoef k0 = pair_coef(G, -1.0), k1 = pair_coef(G, 1.0);
+        switch (tp) {
+        case 8: reg_pair<0, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+        case 9: reg_pair<1, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+#if KBWH_TILE_BITS > 11
+        case 10: reg_pair<2, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+        default: reg_pair<3, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+#else
+        default: reg_pair<2, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+#endif
+        }
+        return;
+    }
+    const PairCoef k = pair_coef(G, ((tid >> tp) & 1u) ? 1.0 : -1.0);
+    thread_pair<SWAP, CTRL>(v, lds, tid, tp, cm, k);
+}
+
+__device__ __forceinline__ void apply_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
+    const bool swap = G.kind == KX;
+    if (G.cm == 0) {
+        if (swap)
+            apply_pair_t<true, false>(v, lds, tid, G);
+        else
+            apply_pair_t<false, false>(v, lds, tid, G);
+    } else {
+        if (swap)
+            apply_pair_t<true, true>(v, lds, tid, G);
+        else
+            apply_pair_t<false, true>(v, lds, tid, G);
+    }
+}
+
+// Z, P and RZ: each amplitude is multiplied by the phase of its half
+// (dense_v2.rs:190-191, 215-216, 271-272); Z and P leave the |0> half alone.
+template <bool CTRL>
+__device__ __forceinline__ void diag_all(dv2 (&v)[TR], uint32_t tid, uint32_t cm, dv2 ph) {
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
-        uint64_t o = dt;
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-            if ((r >> b) & 1) o |= ts.qbit[8 + b];
-        const d2v x = __builtin_nontemporal_load((const d2v *)(a + o));
-        v[r] = make_double2(x.x, x.y);
+        const dv2 out = cmul(v[r], ph);
+        if constexpr (CTRL) {
+            const uint32_t j = tid | ((uint32_t)r << 8);
+            v[r] = ((j & cm) == cm) ? out : v[r];
+        } else {
+            v[r] = out;
+        }
     }
+}
+
+// target on register bit RB: which slots are |1> is known at compile time
+template <int RB, bool CTRL>
+__device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm, bool rz, dv2 ph0,
+                                         dv2 ph1) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const bool one = (r >> RB) & 1;
+        if (!one && !rz) continue;
+        const dv2 out = cmul(v[r], one ? ph1 : ph0);
+        if constexpr (CTRL) {
+            const uint32_t j = tid | ((uint32_t)r << 8);
+            v[r] = ((j & cm) == cm) ? out : v[r];
+        } else {
+            v[r] = out;
+        }
+    }
+}
+
+template <bool CTRL>
+__device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
+    const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
+    const uint32_t cm = G.cm, tp = G.tpos;
+    const bool rz = G.kind == KRZ;
+    if (tp == kOutside || tp < 8) {
+        // the target bit is fixed per tile (outside Q) or per thread (lane/wave bit)
+        const bool one = tp == kOutside ? (base & G.otb) != 0 : ((tid >> tp) & 1u) != 0;
+        if (!one && !rz) return;
+        diag_all<CTRL>(v, tid, cm, one ? ph1 : ph0);
+        return;
+    }
+    switch (tp) {
+    case 8: diag_reg<0, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+    case 9: diag_reg<1, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+#if KBWH_TILE_BITS > 11
+    case 10: diag_reg<2, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+    default: diag_reg<3, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+#else
+    default: diag_reg<2, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+#endif
+    }
+}
+
+__device__ __forceinline__ void apply_diag(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
+    if (G.cm == 0)
+        apply_diag_t<false>(v, base, tid, G);
+    else
+        apply_diag_t<true>(v, base, tid, G);
+}
+
+// The tile number spread over the bits outside Q (a 0 inserted at each position of Q).
+__device__ __forceinline__ uint64_t tile_base(uint64_t k, const TileSpec &ts) {
+#pragma unroll
+    for (int j = 0; j < TL; ++j) {
+        const uint64_t lo = k & ts.qlo[j];
+        k = ((k ^ lo) << 1) | lo;
+    }
+    return k;
+}
+
+__device__ __forceinline__ uint64_t slot_offset(uint64_t dt, int r, const TileSpec &ts) {
+    uint64_t o = dt;
+#pragma unroll
+    for (int b = 0; b < TL - 8; ++b)
+        if ((r >> b) & 1) o |= ts.qbit[8 + b];
+    return o;
+}
+
+// s_waitcnt encodings (gfx9 family): wait for vector-memory counter 0 only /
+// for the LDS counter 0 only.
+constexpr int kWaitVm0 = 0x0F70;
+constexpr int kWaitLgkm0 = 0xC07F;
+
+// Stream tile `tile` into the LDS staging buffer with gfx950's 16-byte
+// global_load_lds: each wave's load r fills 64 consecutive slots
+// stage[r*256 + wave*64 + lane], i.e. each thread's own slot r*256 + tid.
+__device__ __forceinline__ void stage_tile(const dv2 *__restrict__ a, dv2 *stage, uint64_t dt,
+                                           const TileSpec &ts, uint32_t tid) {
+    const uint32_t wave_base = tid & ~63u;
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const dv2 *src = a + slot_offset(dt, r, ts);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                         (__attribute__((address_space(3))) void *)(stage + r * TT + wave_base),
+                                         16, 0, 0);
+    }
+}
+
+#if KBWH_TILE_PIPELINE
+// Persistent: workgroup w handles tiles w, w + G, w + 2G, ...  While the
+// gates are applied to the current tile in registers, the next tile streams
+// from HBM into the LDS staging buffer, so HBM traffic and arithmetic overlap.
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_tile(
+    dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles) {
+    __shared__ dv2 stage[TT * TR];
+    __shared__ dv2 lds[TT * kChunk];
+    const uint32_t tid = threadIdx.x;
+    uint64_t tmask = 0;  // the thread's own tile-position bits
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+        if ((tid >> b) & 1u) tmask |= ts.qbit[b];
+    uint32_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    stage_tile(a, stage, tile_base(tile, ts) | tmask, ts, tid);
+    dv2 v[TR];
+    for (; tile < ntiles; tile += gridDim.x) {
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this thread's staged slots have landed
+#pragma unroll
+        for (int r = 0; r < TR; ++r) v[r] = stage[r * TT + tid];
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // slots read before they are refilled
+        const uint32_t next = tile + gridDim.x;
+        if (next < ntiles) stage_tile(a, stage, tile_base(next, ts) | tmask, ts, tid);
+        const uint64_t base = tile_base(tile, ts);
+        for (uint32_t i = 0; i < ng; ++i) {
+            const TGate G = gates[i];
+            if ((base & G.ocm) != G.ocm) continue;  // a control outside the tile is 0 here
+            if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
+                apply_diag(v, base, tid, G);
+            else
+                apply_pair(v, lds, tid, G);
+        }
+        const uint64_t dt = base | tmask;
+#pragma unroll
+        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(dt, r, ts));
+    }
+}
+
+#else
+// One tile per workgroup: load the tile into registers, apply the gates,
+// store it back; overlap comes from the other resident workgroups.
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_tile(
+    dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles) {
+    __shared__ dv2 lds[TT * kChunk];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    uint64_t tmask = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+        if ((tid >> b) & 1u) tmask |= ts.qbit[b];
+    const uint64_t base = tile_base(tile, ts);
+    const uint64_t dt = base | tmask;
+    dv2 v[TR];
+#pragma unroll
+    for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
     for (uint32_t i = 0; i < ng; ++i) {
         const TGate G = gates[i];
         if ((base & G.ocm) != G.ocm) continue;  // a control outside the tile is 0 here
-        switch (G.kind) {
-        case KX: apply_pair<KX>(v, lds, tid, G); break;
-        case KY: apply_pair<KY>(v, lds, tid, G); break;
-        case KH: apply_pair<KH>(v, lds, tid, G); break;
-        case KRX: apply_pair<KRX>(v, lds, tid, G); break;
-        case KRY: apply_pair<KRY>(v, lds, tid, G); break;
-        case KZ: apply_diag<KZ>(v, base, tid, G); break;
-        case KP: apply_diag<KP>(v, base, tid, G); break;
-        default: apply_diag<KRZ>(v, base, tid, G); break;
-        }
+        if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
+            apply_diag(v, base, tid, G);
+        else
+            apply_pair(v, lds, tid, G);
     }
 #pragma unroll
-    for (int r = 0; r < TR; ++r) {
-        uint64_t o = dt;
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-            if ((r >> b) & 1) o |= ts.qbit[8 + b];
-        d2v x;
-        x.x = v[r].x;
-        x.y = v[r].y;
-        __builtin_nontemporal_store(x, (d2v *)(a + o));
-    }
+    for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(dt, r, ts));
 }
+
+#endif
 
 inline bool is_diagonal(int kind) { return kind == KZ || kind == KP || kind == KRZ; }
 
@@ -270,18 +447,24 @@ void assign_positions(uint32_t n, const Pass &p, int pos_of[64], uint32_t q_at[T
                      [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
                          return x.first < y.first;
                      });
-    const int order[9] = {8, 9, 10, 11, 3, 4, 5, 6, 7};
+    int order[TL - 3], fill_order[TL - 3];
+    {
+        int o = 0;
+        for (int b = 8; b < TL; ++b) order[o++] = b;  // register bits first
+        for (int b = 3; b < 8; ++b) order[o++] = b;   // then lane bits 3..5, wave bits 6..7
+        for (int b = 0; b < TL - 3; ++b) fill_order[b] = 3 + b;
+    }
     size_t k = 0;
     for (; k < freq.size(); ++k) {
         q_at[order[k]] = freq[k].second;
         pos_of[freq[k].second] = order[k];
     }
+    // free positions take the lowest unused qubits (n >= TL leaves enough)
     uint32_t filler = 3;
-    const int fill_order[9] = {3, 4, 5, 6, 7, 8, 9, 10, 11};
-    for (int i = 0; i < 9; ++i) {
+    for (int i = 0; i < TL - 3; ++i) {
         const int b = fill_order[i];
         if (q_at[b] != 0xFFFFFFFFu) continue;
-        while (filler < n && pos_of[filler] >= 0) ++filler;
+        while (pos_of[filler] >= 0) ++filler;
         q_at[b] = filler;
         pos_of[filler] = b;
         ++filler;
@@ -335,7 +518,8 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
             L.spec.qbit[b] = 1ull << q_at[b];
             qmask |= 1ull << q_at[b];
         }
-        L.spec.ins = make_insert(qmask);
+        const Insert ins = make_insert(qmask);
+        for (int j = 0; j < TL; ++j) L.spec.qlo[j] = ins.mlo[j];
         L.first = (uint32_t)gates.size();
         for (const PhysOp &o : cur.ops) {
             TGate g;
@@ -351,12 +535,17 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
                     g.ocm |= 1ull << q;
             }
             const Coef k = make_coef(o.kind, o.angle);
-            g.c = k.c;
-            g.s = k.s;
-            g.ph0x = k.ph0.x;
-            g.ph0y = k.ph0.y;
-            g.ph1x = k.ph1.x;
-            g.ph1y = k.ph1.y;
+            g.d = 1.0;
+            switch (o.kind) {
+            case KX: g.bu = 1.0; break;
+            case KY: g.crs = -1.0; g.cis = 1.0; break;
+            case KH: g.as = -1.0; g.bu = 1.0; g.d = 0.70710678118654752440; break;
+            case KRX: g.au = k.c; g.cru = k.s; g.ciu = -k.s; break;
+            case KRY: g.au = k.c; g.bs = k.s; break;
+            case KZ: g.ph0x = 1.0; g.ph1x = -1.0; g.ph1y = -0.0; break;  // dense_v2.rs:191
+            case KP: g.ph0x = 1.0; g.ph1x = k.ph1.x; g.ph1y = k.ph1.y; break;
+            default: g.ph0x = k.ph0.x; g.ph0y = k.ph0.y; g.ph1x = k.ph1.x; g.ph1y = k.ph1.y; break;
+            }
             gates.push_back(g);
         }
         L.count = (uint32_t)cur.ops.size();
@@ -379,10 +568,15 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         hipError_t e = hipMemcpyAsync(s->tile_gates, gates.data(), bytes, hipMemcpyHostToDevice, s->stream);
         if (e != hipSuccess) return hip_status(e, "tile gate upload");
         s->last_passes += launches.size();
-        const uint32_t grid = (uint32_t)(1ull << (n - TL));
+        const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) cus = 256;
+        const uint32_t grid = KBWH_TILE_PIPELINE
+                                  ? std::min<uint32_t>(ntiles, (uint32_t)std::max(cus, 1) * kTilesPerCU)
+                                  : ntiles;
         const TGate *dg = (const TGate *)s->tile_gates;
         for (const Launch &L : launches) {
-            k_tile<<<grid, TT, 0, s->stream>>>(amp, L.spec, dg + L.first, L.count);
+            k_tile<<<grid, TT, 0, s->stream>>>((dv2 *)amp, L.spec, dg + L.first, L.count, ntiles);
             e = hipGetLastError();
             if (e != hipSuccess) return hip_status(e, "k_tile launch");
         }

@@ -30,8 +30,8 @@ METRIC = "30-qubit c128 gates/s + HBM TB/s; adjoint grads/s vs plain at 1e-9"
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE at 30 qubits, each
 # scaled by the same counter on a 1 GiB device copy (fetch x2.0, write x1.0).
-TRAFFIC_PAIR = 34361835520.0
-TRAFFIC_TILE = None
+TRAFFIC_PAIR = 34362304919.0
+TRAFFIC_TILE = 34363360148.0
 
 
 def parse():
@@ -43,15 +43,19 @@ def parse():
     ap.add_argument("--layers", type=int, default=20)
     ap.add_argument("--mode", choices=["per_gate", "tiled", "best"], default="best")
     ap.add_argument("--cpu-gates", type=int, default=150, help="oracle sample size (gates); 0 = skip")
+    ap.add_argument("--legs", default="tiled,per_gate,exp,adjoint,cpu",
+                    help="comma list of legs: tiled, per_gate, exp, adjoint, cpu")
     ap.add_argument("--pmc", action="store_true",
                     help="counter run: a 1 GiB torch copy then 4 pair passes (for rocprofv3 --pmc)")
     return ap.parse_args()
 
 
 def pmc_run(n):
+    """A 1 GiB device copy (the counter scale), 4 uncontrolled pair passes and
+    the first 3 tiled passes of the brickwork circuit."""
     import torch
 
-    from kbw_amd import DenseHip
+    from kbw_amd import DenseHip, RUN_TILED, circuits
 
     a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
     a.fill_(1.0)
@@ -61,8 +65,68 @@ def pmc_run(n):
     s = DenseHip(n)
     for t in (3, 11, 20, 29):
         s.hadamard(t)
+    ops, _ = circuits.brickwork(n, 20, seed=1)
+    s.run(ops[:60], mode=RUN_TILED)
     s.synchronize()
-    print(json.dumps({"pmc_run": True, "copy_bytes": 1 << 30, "pair_pass_bytes": 32 << n}), flush=True)
+    print(json.dumps({"pmc_run": True, "copy_bytes": 1 << 30, "pass_bytes": 32 << n,
+                      "tiled_passes": s.run_stats()[1]}), flush=True)
+
+
+def adjoint_leg(subset=4, batch=256):
+    """24-qubit, 12-layer RY/RZ + CZ-ring ansatz (576 parameters) with the XXZ
+    chain: adjoint gradients/s, against the plain per-gate path's parameter
+    shift on `subset` parameters (2 circuit runs + 2 expectation values each),
+    and a batch of 16-qubit parameter sets."""
+    import time as _t
+
+    import numpy as np
+
+    from kbw_amd import DenseHip, RUN_PER_GATE, RUN_TILED, circuits
+    from kbw_amd import autograd as kag
+
+    n = 24
+    ops, params = circuits.hea(n, 12, seed=7)
+    ham = circuits.xxz(n, seed=8)
+    sim = DenseHip(n)
+    sim.adjoint_gradient(ops, params, ham)
+    reps = 3
+    t0 = _t.perf_counter()
+    for _ in range(reps):
+        e, g = sim.adjoint_gradient(ops, params, ham)
+    t_adj = (_t.perf_counter() - t0) / reps
+    idx = list(np.linspace(0, len(params) - 1, subset).astype(int))
+    kag.energy(sim, ops, params, ham, RUN_PER_GATE)
+    t0 = _t.perf_counter()
+    ps = kag.parameter_shift(sim, ops, params, ham, indices=idx, mode=RUN_PER_GATE)
+    t_ps = (_t.perf_counter() - t0) / len(idx)
+    t0 = _t.perf_counter()
+    ps_t = kag.parameter_shift(sim, ops, params, ham, indices=idx, mode=RUN_TILED)
+    t_pst = (_t.perf_counter() - t0) / len(idx)
+    err = float(np.max(np.abs(g[idx] - ps)))
+    sim.close()
+    out = {"workload": "hea-24q-12layers-576params-xxz23", "energy": e,
+           "adjoint_s": t_adj, "grads_per_s": len(params) / t_adj,
+           "plain_shift_s_per_param": t_ps, "plain_shift_grads_per_s": 1.0 / t_ps,
+           "speedup_vs_plain_shift": t_ps * len(params) / t_adj,
+           "tiled_shift_grads_per_s": 1.0 / t_pst,
+           "max_abs_diff_vs_shift": err, "shift_subset": [int(i) for i in idx],
+           "tolerance": 1e-9, "within_tolerance": err <= 1e-9}
+    if batch:
+        nb = 16
+        bops, _ = circuits.hea(nb, 12, seed=9)
+        bham = circuits.xxz(nb, seed=10)
+        rng = np.random.default_rng(11)
+        sets = rng.uniform(-np.pi, np.pi, size=(batch, 12 * 2 * nb))
+        bs = DenseHip(nb)
+        bs.adjoint_gradient(bops, sets[0], bham)
+        t0 = _t.perf_counter()
+        for p in sets:
+            bs.adjoint_gradient(bops, p, bham)
+        t_b = _t.perf_counter() - t0
+        bs.close()
+        out["batch16"] = {"sets": batch, "params_per_set": sets.shape[1], "s": t_b,
+                          "grads_per_s": batch * sets.shape[1] / t_b}
+    return out
 
 
 def cpu_baseline(n, ops, ngates):
@@ -101,14 +165,16 @@ def main():
     kinds = ops["kind"].astype(int)
     tgt = ops["target"].astype(np.uint64)
     ctrl = ops["ctrl_mask"].astype(np.uint64) & ~(np.uint64(1) << tgt)
-    is_pair = np.isin(kinds, [_lib.X, _lib.Y, _lib.H, _lib.RX, _lib.RY]) & (ctrl == 0)
+    # per-gate roofline kernel: the H pass, k_pair<KH> (never controlled in this circuit)
+    is_pair = (kinds == _lib.H) & (ctrl == 0)
     alg_bytes = circuits.circuit_bytes(n, ops)
 
     sim = DenseHip(n)
     stream = torch.cuda.ExternalStream(sim.stream())
     lib, h = sim._lib, sim.handle
-    want_pg = args.mode in ("per_gate", "best")
-    want_tiled = args.mode in ("tiled", "best")
+    legs = set(args.legs.split(","))
+    want_pg = args.mode in ("per_gate", "best") and "per_gate" in legs
+    want_tiled = args.mode in ("tiled", "best") and "tiled" in legs
 
     def per_gate_step():
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(nops + 1)]
@@ -160,7 +226,7 @@ def main():
               "hbm_TBps_algorithmic": alg_bytes * pg_steps / el / 1e12,
               "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                            "frac": ach / HBM_PEAK_GBPS, "traffic": TRAFFIC_PAIR,
-                           "kernel": "k_pair<uncontrolled X/Y/H/RX/RY>", "launch_ms": pair_ms,
+                           "kernel": "k_pair<KH> (uncontrolled H pass)", "launch_ms": pair_ms,
                            "bytes_per_launch": per_launch, "launches": int(is_pair.sum()) * pg_steps}}
         result["per_gate"] = pg
 
@@ -180,27 +246,31 @@ def main():
                        "gates_per_pass": gates / passes})
         result["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                               "frac": ach / HBM_PEAK_GBPS, "traffic": TRAFFIC_TILE,
-                              "kernel": "k_tile (12-qubit tile, gates applied in registers/LDS)",
+                              "kernel": "k_tile (11-qubit register tiles, all gates of a pass)",
                               "launch_ms": launch_ms, "bytes_per_launch": per_launch,
                               "launches": passes * args.steps}
         result["hbm_TBps_algorithmic_per_gate_equivalent"] = alg_bytes * args.steps / el / 1e12
-    else:
+    elif pg is not None:
         result.update({"value": pg["gates_per_s"], "ms_per_step": pg["ms_per_step"]})
         config["run_mode"] = "per_gate"
         result["roofline"] = pg["roofline"]
     result["config"] = config
 
+    result["traffic_note"] = "traffic = FETCH_SIZE*2.0 + WRITE_SIZE*1.0 per launch (scale from a 1 GiB copy)"
     # expectation value of the 64-string Hamiltonian
-    t1 = time.perf_counter()
-    e = sim.exp_value(ham)
-    t_exp = time.perf_counter() - t1
-    result["exp_value"] = {"strings": len(ham.coefficients), "ms": 1e3 * t_exp,
-                           "GBps": len(ham.coefficients) * (16 << n) / t_exp / 1e9, "value": e}
+    if "exp" in legs:
+        t1 = time.perf_counter()
+        e = sim.exp_value(ham)
+        t_exp = time.perf_counter() - t1
+        result["exp_value"] = {"strings": len(ham.coefficients), "ms": 1e3 * t_exp,
+                               "GBps": len(ham.coefficients) * (16 << n) / t_exp / 1e9, "value": e}
     sim.close()
     del sim
     torch.cuda.synchronize()
 
-    if args.cpu_gates > 0:
+    if "adjoint" in legs:
+        result["adjoint"] = adjoint_leg()
+    if args.cpu_gates > 0 and "cpu" in legs:
         result["cpu_baseline"] = cpu_baseline(n, ops, args.cpu_gates)
     print(json.dumps(result), flush=True)
 

@@ -72,6 +72,35 @@ def pmc_run(n):
                       "tiled_passes": s.run_stats()[1]}), flush=True)
 
 
+def block_leg(shots=1_000_000):
+    """33-qubit blocked QFT + 4 brickwork layers (128 GiB, 2^28-amplitude
+    blocks, 5 global qubits) in tiled passes, then 10^6 shots."""
+    import time as _t
+
+    import numpy as np
+
+    from kbw_amd import DenseHip, RUN_TILED, circuits
+
+    n, L = 33, 28
+    ops = circuits.qft_plus_brickwork(n, 4, seed=2)
+    s = DenseHip(n, block=True, block_size=L)
+    s.synchronize()
+    t0 = _t.perf_counter()
+    s.run(ops, mode=RUN_TILED)
+    s.synchronize()
+    t_run = _t.perf_counter() - t0
+    gates, passes = s.run_stats()
+    u = np.random.default_rng(9).random(shots)
+    t0 = _t.perf_counter()
+    s.sample(list(range(n)), shots, u)
+    t_s = _t.perf_counter() - t0
+    out = {"workload": "qft+4layers-33q-block28", "gates": int(gates), "passes": int(passes),
+           "swaps": s.swap_count(), "run_s": t_run, "gates_per_s": gates / t_run,
+           "sample_shots": shots, "sample_s": t_s}
+    s.close()
+    return out
+
+
 def adjoint_leg(subset=4, batch=256):
     """24-qubit, 12-layer RY/RZ + CZ-ring ansatz (576 parameters) with the XXZ
     chain: adjoint gradients/s, against the plain per-gate path's parameter
@@ -270,6 +299,8 @@ def main():
 
     if "adjoint" in legs:
         result["adjoint"] = adjoint_leg()
+    if "c2" in legs:
+        result["block33"] = block_leg()
     if args.cpu_gates > 0 and "cpu" in legs:
         result["cpu_baseline"] = cpu_baseline(n, ops, args.cpu_gates)
     print(json.dumps(result), flush=True)

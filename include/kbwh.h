@@ -148,6 +148,50 @@ int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
                               const double *coeffs, uint32_t num_strings, double *energy,
                               double *grad);
 
+/* ---- batches of circuits (no counterpart in the reference: its
+ * BatchExecution replays one circuit per call on a fresh simulator,
+ * quantum_execution.rs:367-427) --------------------------------------------
+ * Many circuits of uneven qubit counts run in one persistent launch.  Circuit
+ * c uses ops[first_op .. first_op+num_ops) (op.param indexes
+ * params[first_param + op.param]) and the Pauli strings
+ * [first_string .. first_string+num_strings) of the shared Hamiltonian
+ * arrays (string k is paulis/qubits[offsets[k] .. offsets[k+1]) with weight
+ * coeffs[k]).  kbwh_batch_run starts every circuit from |0...0> and writes
+ * <H_c> for each circuit.  The gate structure is planned once at
+ * kbwh_batch_new; only the parameters change between runs.  Angles of
+ * parameterised gates are evaluated on the device, so results agree with the
+ * per-gate path to ~1e-15, not bit for bit. */
+typedef struct kbwh_circuit {
+    uint32_t num_qubits;  /* 1..26 */
+    uint32_t num_strings;
+    uint64_t first_op;
+    uint64_t num_ops;
+    uint32_t first_string;
+    uint32_t first_param;
+} kbwh_circuit;
+
+typedef struct kbwh_batch kbwh_batch;
+
+int32_t kbwh_batch_new(const kbwh_circuit *circuits, uint32_t num_circuits, const kbwh_op *ops,
+                       const uint8_t *paulis, const uint32_t *qubits, const uint32_t *offsets,
+                       const double *coeffs, uint64_t num_params, kbwh_batch **out);
+/* Plans the batch on the host only (no device needed) and checks the plan;
+ * reports gate records, tiled passes and state amplitudes. */
+int32_t kbwh_batch_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kbwh_op *ops,
+                        const uint8_t *paulis, const uint32_t *qubits, const uint32_t *offsets,
+                        const double *coeffs, uint64_t num_params, uint64_t *gates, uint64_t *passes,
+                        uint64_t *amplitudes);
+int32_t kbwh_batch_run(kbwh_batch *b, const double *params, double *exp_values);
+/* enable != 0: the first run is captured in a hipGraph (parameter upload,
+ * coefficient, reset and batch kernels, result download) and later runs
+ * replay it. */
+int32_t kbwh_batch_use_graph(kbwh_batch *b, int32_t enable);
+/* The first `count` amplitudes of circuit `circuit` after the last run. */
+int32_t kbwh_batch_state(kbwh_batch *b, uint32_t circuit, uint64_t count, double *out);
+/* Gate records, tiled passes and amplitude-passes (sum over circuits of 2^n x passes). */
+int32_t kbwh_batch_stats(const kbwh_batch *b, uint64_t *gates, uint64_t *passes, uint64_t *amplitudes);
+int32_t kbwh_batch_free(kbwh_batch *b);
+
 /* ---- raw access (tests, tooling) ------------------------------------- */
 /* Copies amplitudes [first, first+count) of the physical buffer. */
 int32_t kbwh_read_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, double *out);

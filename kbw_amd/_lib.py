@@ -30,6 +30,11 @@ OP_DTYPE = np.dtype([("kind", "u1"), ("target", "u1"), ("reserved", "<u2"), ("pa
                      ("angle", "<f8"), ("ctrl_mask", "<u8")], align=True)
 assert OP_DTYPE.itemsize == 24
 
+# kbwh_circuit, 32 bytes
+CIRCUIT_DTYPE = np.dtype([("num_qubits", "<u4"), ("num_strings", "<u4"), ("first_op", "<u8"),
+                          ("num_ops", "<u8"), ("first_string", "<u4"), ("first_param", "<u4")], align=True)
+assert CIRCUIT_DTYPE.itemsize == 32
+
 # Every exported symbol: (name, argtypes).  All return int32 except kbwh_last_error.
 _vp, _u32, _u64, _i32, _f64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_double
 _p = C.c_void_p
@@ -56,6 +61,13 @@ SIGNATURES = {
     "kbwh_run_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64)],
     "kbwh_stream": [_vp, C.POINTER(_vp)],
     "kbwh_device_buffer": [_vp, C.POINTER(_vp), C.POINTER(_u64)],
+    "kbwh_batch_new": [_p, _u32, _p, _p, _p, _p, _p, _u64, C.POINTER(_vp)],
+    "kbwh_batch_plan": [_p, _u32, _p, _p, _p, _p, _p, _u64, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)],
+    "kbwh_batch_run": [_vp, _p, _p],
+    "kbwh_batch_use_graph": [_vp, _i32],
+    "kbwh_batch_state": [_vp, _u32, _u64, _p],
+    "kbwh_batch_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)],
+    "kbwh_batch_free": [_vp],
     "kbwh_build_info": [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)],
     "kbwh_last_error": [],
 }

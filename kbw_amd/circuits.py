@@ -183,6 +183,24 @@ def xxz(n, seed):
     return Hamiltonian(strings, coeffs)
 
 
+def hea_batch(num=4096, seed=12, n_low=10, n_high=20, layers=8):
+    """The batched workload: `num` hardware-efficient-ansatz circuits with
+    qubit counts U{n_low..n_high}, `layers` layers of RY then RZ on every
+    qubit (angles U[-pi, pi)) and a CZ ring per layer, each ending with the
+    expectation value of an open transverse-field Ising chain (J = 1,
+    h_i ~ N(1, 0.1)).  Returns (circuits, params) for kbw_amd.batch.Batch:
+    circuits[i] = (n, ops, hamiltonian, num_params)."""
+    rng = np.random.default_rng(seed)
+    circuits, params = [], []
+    for i in range(num):
+        n = int(rng.integers(n_low, n_high + 1))
+        ops, p = hea(n, layers, seed=int(rng.integers(0, 2 ** 31)))
+        ham = tfim(n, seed=int(rng.integers(0, 2 ** 31)))
+        circuits.append((n, ops, ham, len(p)))
+        params.append(p)
+    return circuits, np.concatenate(params)
+
+
 def random_circuit(rng, n, ngates, max_controls=3):
     """Random gates of every kind with random targets and controls (tests)."""
     ops = OpList()

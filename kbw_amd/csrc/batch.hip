@@ -1,0 +1,521 @@
+// Batches of circuits in one persistent launch (kbwh_batch_*).
+//
+// Each circuit of a batch owns a slice of one HBM buffer (2^max(n, TL)
+// amplitudes; a circuit narrower than a tile runs with idle extra qubits that
+// stay |0>, which leaves its amplitudes unchanged).  Its gates are planned
+// once on the host into tiled passes (the same planner and gate application
+// as KBWH_RUN_TILED).  A run is three kernels:
+//   k_coef   -- the coefficients of every parameterised gate from the
+//               parameter vector (device sincos), once per run;
+//   k_reset  -- the work counter back to 0 (a kernel, not a memset, so the
+//               run can be captured in a hipGraph);
+//   k_batch  -- persistent: every workgroup takes whole circuits from the
+//               counter, largest first, prepares |0...0>, applies each pass
+//               to every tile of its circuit, then reduces the circuit's
+//               Hamiltonian; a workgroup exits when the counter runs past the
+//               last circuit, so every wave reaches the exit.
+// With graphs enabled, the parameter upload, the three kernels and the result
+// download are captured once and replayed by later runs.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "state.h"
+#include "tile_core.h"
+
+#pragma clang fp contract(off)
+
+using namespace kbwh;
+using namespace kbwh::tile;
+
+namespace {
+
+constexpr uint32_t kMaxBatchQubits = 26;
+
+struct BCirc {
+    uint64_t state_off;   // first amplitude of the circuit's slice
+    uint32_t n_eff;       // max(n, TL)
+    uint32_t first_pass, num_passes;
+    uint32_t first_string, num_strings;
+    uint32_t pad;
+};
+
+struct BPass {
+    TileSpec ts;
+    uint32_t first_gate, num_gates;
+};
+
+struct BString {
+    uint64_t x, z;
+    uint32_t ny, pad;
+    double coeff;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *ws) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < TT / 64; ++w) t += ws[w];
+    __syncthreads();
+    return t;  // the same value in every thread
+}
+
+// coefficients of parameterised gates: the template's pad field holds the
+// absolute parameter index (or -1), the live copy gets the coefficients
+__global__ __launch_bounds__(256) void k_coef(const TGate *__restrict__ tmpl, TGate *__restrict__ live,
+                                              uint64_t ngates, const double *__restrict__ params) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ngates) return;
+    TGate g = tmpl[i];
+    const int32_t p = (int32_t)g.pad;
+    if (p >= 0) {
+        const double th = params[p];
+        double s, c;
+        switch (g.kind) {
+        case KRX:
+            sincos(th / 2.0, &s, &c);
+            g.au = c;
+            g.cru = s;
+            g.ciu = -s;
+            break;
+        case KRY:
+            sincos(th / 2.0, &s, &c);
+            g.au = c;
+            g.bs = s;
+            break;
+        case KP:
+            sincos(th, &s, &c);
+            g.ph0x = 1.0;
+            g.ph1x = 1.0 * c;
+            g.ph1y = 1.0 * s;
+            break;
+        default:  // KRZ
+            sincos(-th / 2.0, &s, &c);
+            g.ph0x = 1.0 * c;
+            g.ph0y = 1.0 * s;
+            sincos(th / 2.0, &s, &c);
+            g.ph1x = 1.0 * c;
+            g.ph1y = 1.0 * s;
+            break;
+        }
+    }
+    live[i] = g;
+}
+
+__global__ void k_reset(uint32_t *counter) { *counter = 0; }
+
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_batch(
+    dv2 *__restrict__ states, const BCirc *__restrict__ circs, const uint32_t *__restrict__ order,
+    uint32_t ncirc, const BPass *__restrict__ passes, const TGate *__restrict__ gates,
+    const BString *__restrict__ strings, double *__restrict__ results, uint32_t *__restrict__ counter) {
+    __shared__ dv2 lds[TT * kChunk];
+    __shared__ double ws[TT / 64];
+    __shared__ uint32_t s_item;
+    const uint32_t tid = threadIdx.x;
+    // Thread 0 fetches the next item in ONE place per iteration (together
+    // with the result store) and every wave reads it back as a uniform value:
+    // with a second `tid == 0` block at the loop top the compiler merged the
+    // two into a loop nest in which lanes 1..63 re-entered the item read
+    // without thread 0 fetching, so the item never advanced.
+    if (tid == 0) s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    for (;;) {
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item >= ncirc) break;
+        const uint32_t ci = order[item];
+        const BCirc C = circs[ci];
+        dv2 *a = states + C.state_off;
+        const uint64_t size = 1ull << C.n_eff;
+        for (uint64_t i = tid; i < size; i += TT) a[i] = mk2(i == 0 ? 1.0 : 0.0, 0.0);
+        __threadfence_block();
+        __syncthreads();
+        const uint32_t ntiles = 1u << (C.n_eff - TL);
+        for (uint32_t p = 0; p < C.num_passes; ++p) {
+            const BPass &P = passes[C.first_pass + p];
+            uint64_t tmask = 0;
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+                if ((tid >> b) & 1u) tmask |= P.ts.qbit[b];
+            for (uint32_t t = 0; t < ntiles; ++t) {
+                const uint64_t base = tile_base(t, P.ts);
+                const uint64_t dt = base | tmask;
+                dv2 v[TR];
+#pragma unroll
+                for (int r = 0; r < TR; ++r) v[r] = a[slot_offset(dt, r, P.ts)];
+                for (uint32_t i = 0; i < P.num_gates; ++i) {
+                    const TGate G = gates[P.first_gate + i];
+                    if ((base & G.ocm) != G.ocm) continue;
+                    if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
+                        apply_diag(v, base, tid, G);
+                    else
+                        apply_pair(v, lds, tid, G);
+                }
+#pragma unroll
+                for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, P.ts)] = v[r];
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+        // <H> = sum_k coeff_k <P_k>, in string order (quantum_execution.rs:155-157)
+        double total = 0.0;
+        for (uint32_t k = 0; k < C.num_strings; ++k) {
+            const BString S = strings[C.first_string + k];
+            double acc = 0.0;
+            if (S.x == 0) {
+                for (uint64_t i = tid; i < size; i += TT) {
+                    const dv2 x = a[i];
+                    const double w = x.x * x.x + x.y * x.y;
+                    acc += (__popcll(i & S.z) & 1) ? -w : w;
+                }
+            } else {
+                const uint64_t hmlo = (1ull << __builtin_ctzll(S.x)) - 1;
+                for (uint64_t k2 = tid; k2 < (size >> 1); k2 += TT) {
+                    const uint64_t lo = k2 & hmlo;
+                    const uint64_t i = ((k2 ^ lo) << 1) | lo, j = i ^ S.x;
+                    const dv2 ai = a[i], aj = a[j];
+                    const double wr = aj.x * ai.x + aj.y * ai.y;
+                    const double wi = aj.x * ai.y - aj.y * ai.x;
+                    const uint32_t q = (S.ny + 2u * (uint32_t)(__popcll(i & S.z) & 1)) & 3u;
+                    acc += 2.0 * ((q == 0) ? wr : (q == 1) ? -wi : (q == 2) ? -wr : wi);
+                }
+            }
+            total += block_sum(acc, ws) * S.coeff;
+        }
+        if (tid == 0) {
+            results[ci] = total;
+            s_item = atomicAdd(counter, 1u);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+struct kbwh_batch {
+    uint32_t ncirc = 0;
+    uint64_t nparams = 0;
+    uint64_t ngates = 0;
+    hipStream_t stream = nullptr;
+    dv2 *states = nullptr;
+    BCirc *d_circs = nullptr;
+    uint32_t *d_order = nullptr;
+    BPass *d_passes = nullptr;
+    TGate *d_tmpl = nullptr, *d_live = nullptr;
+    BString *d_strings = nullptr;
+    double *d_params = nullptr, *d_results = nullptr;
+    uint32_t *d_counter = nullptr;
+    double *h_params = nullptr, *h_results = nullptr;  // pinned
+    std::vector<BCirc> circs;
+    uint32_t npasses = 0;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t grid = 0;
+};
+
+namespace {
+
+int32_t enqueue_run(kbwh_batch *b) {
+    hipStream_t st = b->stream;
+    hipError_t e = hipSuccess;
+    if (b->nparams)
+        e = hipMemcpyAsync(b->d_params, b->h_params, b->nparams * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && b->ngates) {
+        k_coef<<<(uint32_t)((b->ngates + 255) / 256), 256, 0, st>>>(b->d_tmpl, b->d_live, b->ngates, b->d_params);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        k_reset<<<1, 1, 0, st>>>(b->d_counter);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        k_batch<<<b->grid, TT, 0, st>>>(b->states, b->d_circs, b->d_order, b->ncirc, b->d_passes, b->d_live,
+                                       b->d_strings, b->d_results, b->d_counter);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->h_results, b->d_results, b->ncirc * 8, hipMemcpyDeviceToHost, st);
+    return hip_status(e, "batch run");
+}
+
+void destroy(kbwh_batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void *dev[] = {b->states, b->d_circs, b->d_order, b->d_passes, b->d_tmpl, b->d_live,
+                   b->d_strings, b->d_params, b->d_results, b->d_counter};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (b->h_params) (void)hipHostFree(b->h_params);
+    if (b->h_results) (void)hipHostFree(b->h_results);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+template <typename T>
+hipError_t upload(T **dst, const std::vector<T> &src) {
+    const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e == hipSuccess && !src.empty()) e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+}  // namespace
+
+namespace {
+
+struct Plan {
+    std::vector<BCirc> circs;
+    std::vector<BPass> passes;
+    std::vector<TGate> tmpl;
+    std::vector<BString> strings;
+    uint64_t state_total = 0;
+};
+
+// Plans every circuit's tiled passes on the host (no device needed) and
+// checks the plan: every non-diagonal gate's target is a tile position, and
+// every pass's tile holds TL distinct qubits of the circuit's width.
+int32_t build_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kbwh_op *ops,
+                   const uint8_t *paulis, const uint32_t *qubits, const uint32_t *offsets,
+                   const double *coeffs, uint64_t num_params, Plan &pl) {
+    std::vector<BCirc> &circs = pl.circs;
+    std::vector<BPass> &passes = pl.passes;
+    std::vector<TGate> &tmpl = pl.tmpl;
+    std::vector<BString> &strings = pl.strings;
+    circs.resize(num_circuits);
+    uint64_t &state_total = pl.state_total;
+    for (uint32_t c = 0; c < num_circuits; ++c) {
+        const kbwh_circuit &K = circuits[c];
+        const uint32_t n = K.num_qubits;
+        if (n == 0 || n > kMaxBatchQubits) return set_error(KBWH_ERR_EXECUTION_FAILED, "batch circuit size out of range");
+        if (K.num_ops && !ops) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null op array");
+        const uint32_t n_eff = std::max<uint32_t>(n, TL);
+        BCirc &B = circs[c];
+        B.state_off = state_total;
+        B.n_eff = n_eff;
+        state_total += 1ull << n_eff;
+        B.first_pass = (uint32_t)passes.size();
+        // plan the circuit's passes (physical = logical qubits)
+        Pass cur;
+        auto close = [&]() {
+            if (cur.ops.empty()) return;
+            int pos_of[64];
+            uint32_t q_at[TL];
+            assign_positions(n_eff, cur, pos_of, q_at);
+            BPass P;
+            P.ts = make_tile_spec(q_at);
+            P.first_gate = (uint32_t)tmpl.size();
+            P.num_gates = (uint32_t)cur.ops.size();
+            for (size_t i = 0; i < cur.ops.size(); ++i) tmpl.push_back(make_tgate(cur.ops[i], pos_of, n_eff));
+            passes.push_back(P);
+            cur = Pass();
+        };
+        std::vector<int32_t> pidx;
+        for (uint64_t i = K.first_op; i < K.first_op + K.num_ops; ++i) {
+            const kbwh_op &op = ops[i];
+            if (op.kind > KRZ || op.target >= n || (op.ctrl_mask >> n))
+                return set_error(KBWH_ERR_INVALID_ARGUMENT, "batch op out of range");
+            int32_t gp = -1;
+            if (op.param >= 0) {
+                if (!(op.kind == KRX || op.kind == KRY || op.kind == KRZ || op.kind == KP))
+                    return set_error(KBWH_ERR_INVALID_ARGUMENT, "parameterised ops must be RX, RY, RZ or P");
+                const uint64_t pi = (uint64_t)K.first_param + (uint64_t)op.param;
+                if (pi >= num_params) return set_error(KBWH_ERR_INVALID_ARGUMENT, "param index out of range");
+                gp = (int32_t)pi;
+            }
+            const PhysOp po{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), op.angle};
+            if (!fits(cur, po)) {
+                // remember parameter indices of the pass being closed
+                close();
+            }
+            add_op(cur, po);
+            pidx.push_back(gp);
+        }
+        close();
+        B.num_passes = (uint32_t)passes.size() - B.first_pass;
+        // attach parameter indices to this circuit's gate records, in order
+        size_t gi = passes.empty() || B.num_passes == 0 ? tmpl.size() : passes[B.first_pass].first_gate;
+        for (size_t k = 0; k < pidx.size(); ++k) tmpl[gi + k].pad = (uint32_t)pidx[k];
+        B.first_string = (uint32_t)strings.size();
+        B.num_strings = K.num_strings;
+        for (uint32_t k = K.first_string; k < K.first_string + K.num_strings; ++k) {
+            if (!offsets || !coeffs) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null Hamiltonian arrays");
+            BString S{0, 0, 0, 0, coeffs[k]};
+            uint64_t seen = 0;
+            for (uint32_t j = offsets[k]; j < offsets[k + 1]; ++j) {
+                const uint32_t q = qubits[j];
+                if (q >= n) return set_error(KBWH_ERR_INVALID_ARGUMENT, "Pauli qubit out of range");
+                if (seen >> q & 1ull) return set_error(KBWH_ERR_INVALID_ARGUMENT, "qubit repeated inside one Pauli string");
+                seen |= 1ull << q;
+                const uint64_t bit = 1ull << q;
+                if (paulis[j] == KBWH_PAULI_X) S.x |= bit;
+                else if (paulis[j] == KBWH_PAULI_Y) { S.x |= bit; S.z |= bit; S.ny++; }
+                else if (paulis[j] == KBWH_PAULI_Z) S.z |= bit;
+                else return set_error(KBWH_ERR_INVALID_ARGUMENT, "unknown Pauli letter");
+            }
+            strings.push_back(S);
+        }
+    }
+    // plan checks
+    for (const BCirc &B : circs) {
+        if (B.n_eff < TL || B.n_eff > kMaxBatchQubits) return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: width");
+        for (uint32_t p = B.first_pass; p < B.first_pass + B.num_passes; ++p) {
+            const BPass &P = passes[p];
+            uint64_t seen = 0;
+            for (int b = 0; b < TL; ++b) {
+                const uint64_t q = P.ts.qbit[b];
+                if (!q || (q & (q - 1)) || (q >> B.n_eff) || (seen & q))
+                    return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: tile qubits");
+                seen |= q;
+            }
+            for (uint32_t i = P.first_gate; i < P.first_gate + P.num_gates; ++i) {
+                const TGate &g = tmpl[i];
+                const bool diag = g.kind == KZ || g.kind == KP || g.kind == KRZ;
+                if (!diag && g.tpos >= (uint32_t)TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: target outside tile");
+                if (g.tpos != kOutside && g.tpos >= (uint32_t)TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: tile position");
+                if (g.cm >> TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: control position");
+            }
+        }
+    }
+    return KBWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t kbwh_batch_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kbwh_op *ops,
+                        const uint8_t *paulis, const uint32_t *qubits, const uint32_t *offsets,
+                        const double *coeffs, uint64_t num_params, uint64_t *gates, uint64_t *passes,
+                        uint64_t *amplitudes) {
+    if (!gates || !passes || !amplitudes || (num_circuits && !circuits))
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    Plan pl;
+    int32_t rc = build_plan(circuits, num_circuits, ops, paulis, qubits, offsets, coeffs, num_params, pl);
+    if (rc) return rc;
+    *gates = pl.tmpl.size();
+    *passes = pl.passes.size();
+    *amplitudes = pl.state_total;
+    return KBWH_OK;
+}
+
+int32_t kbwh_batch_new(const kbwh_circuit *circuits, uint32_t num_circuits, const kbwh_op *ops,
+                       const uint8_t *paulis, const uint32_t *qubits, const uint32_t *offsets,
+                       const double *coeffs, uint64_t num_params, kbwh_batch **out) {
+    if (!out || (num_circuits && !circuits)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_error(KBWH_ERR_NO_DEVICE, "no HIP device visible");
+    Plan pl;
+    int32_t prc = build_plan(circuits, num_circuits, ops, paulis, qubits, offsets, coeffs, num_params, pl);
+    if (prc) return prc;
+    std::vector<BCirc> &circs = pl.circs;
+    std::vector<BPass> &passes = pl.passes;
+    std::vector<TGate> &tmpl = pl.tmpl;
+    std::vector<BString> &strings = pl.strings;
+    const uint64_t state_total = pl.state_total;
+    // largest circuits first: the persistent workgroups balance the tail
+    std::vector<uint32_t> order(num_circuits);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return circs[x].n_eff > circs[y].n_eff; });
+    kbwh_batch *b = new kbwh_batch();
+    b->ncirc = num_circuits;
+    b->nparams = num_params;
+    b->ngates = tmpl.size();
+    b->circs = circs;
+    b->npasses = (uint32_t)passes.size();
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    b->grid = (uint32_t)std::max(1, std::min<int>((int)num_circuits, std::max(cus, 1) * 4));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&b->states, std::max<uint64_t>(state_total, 1) * sizeof(dv2));
+    if (e == hipSuccess) e = upload(&b->d_circs, circs);
+    if (e == hipSuccess) e = upload(&b->d_order, order);
+    if (e == hipSuccess) e = upload(&b->d_passes, passes);
+    if (e == hipSuccess) e = upload(&b->d_tmpl, tmpl);
+    if (e == hipSuccess) e = hipMalloc(&b->d_live, std::max<size_t>(tmpl.size(), 1) * sizeof(TGate));
+    if (e == hipSuccess) e = upload(&b->d_strings, strings);
+    if (e == hipSuccess) e = hipMalloc(&b->d_params, std::max<uint64_t>(num_params, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->d_results, std::max<uint32_t>(num_circuits, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&b->d_counter, 64);
+    if (e == hipSuccess) e = hipHostMalloc(&b->h_params, std::max<uint64_t>(num_params, 1) * 8, 0);
+    if (e == hipSuccess) e = hipHostMalloc(&b->h_results, std::max<uint32_t>(num_circuits, 1) * 8, 0);
+    if (e != hipSuccess) {
+        int32_t rc = hip_status(e, "kbwh_batch_new");
+        destroy(b);
+        return rc;
+    }
+    *out = b;
+    return KBWH_OK;
+}
+
+int32_t kbwh_batch_run(kbwh_batch *b, const double *params, double *exp_values) {
+    if (!b || !exp_values || (b->nparams && !params)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    if (b->nparams) memcpy(b->h_params, params, b->nparams * 8);
+    hipError_t e = hipSuccess;
+    if (b->use_graph) {
+        if (!b->exec) {
+            e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_status(e, "graph capture");
+            int32_t rc = enqueue_run(b);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            if (e2 != hipSuccess) return hip_status(e2, "graph capture end");
+            b->graph = g;
+            e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) return hip_status(e, "graph instantiate");
+        }
+        e = hipGraphLaunch(b->exec, b->stream);
+        if (e != hipSuccess) return hip_status(e, "graph launch");
+    } else {
+        int32_t rc = enqueue_run(b);
+        if (rc) return rc;
+    }
+    e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_status(e, "batch sync");
+    memcpy(exp_values, b->h_results, b->ncirc * 8);
+    return KBWH_OK;
+}
+
+int32_t kbwh_batch_use_graph(kbwh_batch *b, int32_t enable) {
+    if (!b) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null handle");
+    b->use_graph = enable != 0;
+    return KBWH_OK;
+}
+
+int32_t kbwh_batch_state(kbwh_batch *b, uint32_t circuit, uint64_t count, double *out) {
+    if (!b || !out || circuit >= b->ncirc) return set_error(KBWH_ERR_INVALID_ARGUMENT, "bad argument");
+    const BCirc &C = b->circs[circuit];
+    if (count > (1ull << C.n_eff)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "count out of range");
+    hipError_t e = hipMemcpyAsync(out, b->states + C.state_off, count * sizeof(dv2), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    return hip_status(e, "batch state");
+}
+
+int32_t kbwh_batch_stats(const kbwh_batch *b, uint64_t *gates, uint64_t *passes, uint64_t *amplitudes) {
+    if (!b || !gates || !passes || !amplitudes) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    *gates = b->ngates;
+    *passes = b->npasses;
+    uint64_t amps = 0;
+    for (const BCirc &c : b->circs) amps += (1ull << c.n_eff) * c.num_passes;
+    *amplitudes = amps;
+    return KBWH_OK;
+}
+
+int32_t kbwh_batch_free(kbwh_batch *b) {
+    destroy(b);
+    return KBWH_OK;
+}
+
+}  // extern "C"

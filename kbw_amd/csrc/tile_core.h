@@ -1,0 +1,426 @@
+// Tile machinery shared by the tiled pass (tiled.hip) and the batched
+// persistent kernel (batch.hip): the tile layout, the gate record, the exact
+// per-amplitude gate application in registers, and the host-side planner
+// helpers.  Included by those two translation units only.
+#pragma once
+
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "state.h"
+
+#pragma clang fp contract(off)
+
+namespace kbwh {
+namespace tile {
+
+// Tile shape, measured on MI355X for the 30-qubit brickwork circuit: 2^11
+// amplitudes per tile (8 per thread), one tile per workgroup and 4 waves per
+// SIMD beat 2^12-amplitude tiles and an LDS-staged persistent pipeline.
+#ifndef KBWH_TILE_BITS
+#define KBWH_TILE_BITS 11
+#endif
+#ifndef KBWH_TILE_WAVES
+#define KBWH_TILE_WAVES 4
+#endif
+constexpr int TL = KBWH_TILE_BITS;  // tile qubits
+constexpr int TT = 256;             // threads per tile
+constexpr int TR = 1 << (TL - 8);   // amplitudes per thread (register bits = TL - 8)
+constexpr uint32_t kOutside = 255;
+constexpr int kChunk = 4;        // register slots per LDS exchange round (16 KiB of LDS)
+
+struct TGate {
+    uint32_t kind;
+    uint32_t tpos;  // tile position of the target, kOutside if the target is not in Q
+    uint32_t cm;    // controls inside the tile, as tile positions
+    uint32_t pad;
+    uint64_t ocm;   // controls outside the tile, as global bits
+    uint64_t otb;   // the target's global bit (diagonal gates with target outside Q)
+    // pair gates: out coefficients (uniform part u, part times the half's sign s)
+    double au, as, bu, bs, cru, crs, ciu, cis, d;
+    // diagonal gates: the phases of the |0> and |1> halves
+    double ph0x, ph0y, ph1x, ph1y;
+    double pad2;
+};
+static_assert(sizeof(TGate) == 144, "TGate layout");
+
+struct TileSpec {
+    uint64_t qbit[TL];  // qbit[b] = 1 << Q[b]
+    uint64_t qlo[TL];   // (1 << p) - 1 for the positions p of Q in ascending order
+};
+
+enum { KX = 0, KY = 1, KZ = 2, KH = 3, KP = 4, KRX = 5, KRY = 6, KRZ = 7 };
+
+// Amplitudes in the tile kernel are clang vector values (selects on them are
+// value selects; HIP's double2 class would turn a ?: into a select between
+// addresses and push the register array out to scratch memory).
+typedef double dv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ dv2 mk2(double x, double y) {
+    dv2 r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+
+__device__ __forceinline__ dv2 cmul(dv2 a, dv2 b) {
+    dv2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.y * b.x + a.x * b.y;
+    return r;
+}
+
+__device__ __forceinline__ dv2 shfl_xor2(dv2 v, int mask) {
+    return mk2(__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64));
+}
+
+// Every pair closure of the reference (dense_v2.rs:166-263), written for one
+// output amplitude from its own value m and its partner's value o:
+//     out.re = ((A m.re + B o.re) + Cr o.im) * D
+//     out.im = ((A m.im + B o.im) + Ci o.re) * D
+// with coefficients that depend on the half (sigma = +1 for the |1> amplitude,
+// -1 for the |0> one):
+//     X:  A=0     B=1        Cr=0       Ci=0      D=1
+//     Y:  A=0     B=0        Cr=-sigma  Ci=sigma  D=1
+//     H:  A=-sigma B=1       Cr=0       Ci=0      D=1/sqrt2
+//     RX: A=c     B=0        Cr=s       Ci=-s     D=1
+//     RY: A=c     B=sigma*s  Cr=0       Ci=0      D=1
+// Adding a zero product, multiplying by 1 and taking a product by +-1 are
+// exact, and the order of two addends does not change a rounded sum, so each
+// output equals the reference's closure in value (the sign of a zero aside).
+struct PairCoef {
+    double A, B, Cr, Ci, D;
+};
+
+__device__ __forceinline__ PairCoef pair_coef(const TGate &G, double sigma) {
+    PairCoef k;
+    k.A = G.au + G.as * sigma;
+    k.B = G.bu + G.bs * sigma;
+    k.Cr = G.cru + G.crs * sigma;
+    k.Ci = G.ciu + G.cis * sigma;
+    k.D = G.d;
+    return k;
+}
+
+__device__ __forceinline__ dv2 pair_out(dv2 m, dv2 o, const PairCoef &k) {
+    return mk2(((k.A * m.x + k.B * o.x) + k.Cr * o.y) * k.D, ((k.A * m.y + k.B * o.y) + k.Ci * o.x) * k.D);
+}
+
+// Target on register bit RB: both amplitudes of a pair live in one thread.
+// SWAP: the X gate (dense_v2.rs:168-170), a pure exchange.  CTRL: some
+// control lies inside the tile, so only the pairs whose index matches cm change.
+template <int RB, bool SWAP, bool CTRL>
+__device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, uint32_t cm, const PairCoef &k0,
+                                         const PairCoef &k1) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        if ((r >> RB) & 1) continue;
+        const int r1 = r | (1 << RB);
+        const dv2 t0 = v[r], t1 = v[r1];
+        dv2 o0, o1;
+        if constexpr (SWAP) {
+            o0 = t1;
+            o1 = t0;
+        } else {
+            o0 = pair_out(t0, t1, k0);
+            o1 = pair_out(t1, t0, k1);
+        }
+        if constexpr (CTRL) {
+            const uint32_t j0 = tid | ((uint32_t)r << 8);
+            const bool ok = (j0 & cm) == cm;
+            v[r] = ok ? o0 : t0;
+            v[r1] = ok ? o1 : t1;
+        } else {
+            v[r] = o0;
+            v[r1] = o1;
+        }
+    }
+}
+
+template <bool SWAP, bool CTRL>
+__device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, uint32_t tp,
+                                            uint32_t cm, const PairCoef &k) {
+    if (tp < 6) {
+        const int lm = 1 << tp;
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+            const dv2 o = shfl_xor2(v[r], lm);
+            const dv2 out = SWAP ? o : pair_out(v[r], o, k);
+            if constexpr (CTRL) {
+                const uint32_t j = tid | ((uint32_t)r << 8);
+                v[r] = ((j & cm) == cm) ? out : v[r];
+            } else {
+                v[r] = out;
+            }
+        }
+        return;
+    }
+    // wave bits: through LDS, kChunk register slots at a time
+    const uint32_t pt = tid ^ (1u << tp);
+#pragma unroll
+    for (int q = 0; q < TR; q += kChunk) {
+        __syncthreads();  // earlier exchange reads are done
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) lds[tid | (r << 8)] = v[q + r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) {
+            const dv2 o = lds[pt | (r << 8)];
+            const dv2 out = SWAP ? o : pair_out(v[q + r], o, k);
+            if constexpr (CTRL) {
+                const uint32_t j = tid | ((uint32_t)(q + r) << 8);
+                v[q + r] = ((j & cm) == cm) ? out : v[q + r];
+            } else {
+                v[q + r] = out;
+            }
+        }
+    }
+}
+
+template <bool SWAP, bool CTRL>
+__device__ __forceinline__ void apply_pair_t(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
+    const uint32_t tp = G.tpos, cm = G.cm;
+    if (tp >= 8) {
+        const PairCoef k0 = pair_coef(G, -1.0), k1 = pair_coef(G, 1.0);
+        switch (tp) {
+        case 8: reg_pair<0, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+        case 9: reg_pair<1, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+#if KBWH_TILE_BITS > 11
+        case 10: reg_pair<2, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+        default: reg_pair<3, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+#else
+        default: reg_pair<2, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+#endif
+        }
+        return;
+    }
+    const PairCoef k = pair_coef(G, ((tid >> tp) & 1u) ? 1.0 : -1.0);
+    thread_pair<SWAP, CTRL>(v, lds, tid, tp, cm, k);
+}
+
+__device__ __forceinline__ void apply_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
+    const bool swap = G.kind == KX;
+    if (G.cm == 0) {
+        if (swap)
+            apply_pair_t<true, false>(v, lds, tid, G);
+        else
+            apply_pair_t<false, false>(v, lds, tid, G);
+    } else {
+        if (swap)
+            apply_pair_t<true, true>(v, lds, tid, G);
+        else
+            apply_pair_t<false, true>(v, lds, tid, G);
+    }
+}
+
+// Z, P and RZ: each amplitude is multiplied by the phase of its half
+// (dense_v2.rs:190-191, 215-216, 271-272); Z and P leave the |0> half alone.
+template <bool CTRL>
+__device__ __forceinline__ void diag_all(dv2 (&v)[TR], uint32_t tid, uint32_t cm, dv2 ph) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const dv2 out = cmul(v[r], ph);
+        if constexpr (CTRL) {
+            const uint32_t j = tid | ((uint32_t)r << 8);
+            v[r] = ((j & cm) == cm) ? out : v[r];
+        } else {
+            v[r] = out;
+        }
+    }
+}
+
+// target on register bit RB: which slots are |1> is known at compile time
+template <int RB, bool CTRL>
+__device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm, bool rz, dv2 ph0,
+                                         dv2 ph1) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const bool one = (r >> RB) & 1;
+        if (!one && !rz) continue;
+        const dv2 out = cmul(v[r], one ? ph1 : ph0);
+        if constexpr (CTRL) {
+            const uint32_t j = tid | ((uint32_t)r << 8);
+            v[r] = ((j & cm) == cm) ? out : v[r];
+        } else {
+            v[r] = out;
+        }
+    }
+}
+
+template <bool CTRL>
+__device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
+    const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
+    const uint32_t cm = G.cm, tp = G.tpos;
+    const bool rz = G.kind == KRZ;
+    if (tp == kOutside || tp < 8) {
+        // the target bit is fixed per tile (outside Q) or per thread (lane/wave bit)
+        const bool one = tp == kOutside ? (base & G.otb) != 0 : ((tid >> tp) & 1u) != 0;
+        if (!one && !rz) return;
+        diag_all<CTRL>(v, tid, cm, one ? ph1 : ph0);
+        return;
+    }
+    switch (tp) {
+    case 8: diag_reg<0, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+    case 9: diag_reg<1, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+#if KBWH_TILE_BITS > 11
+    case 10: diag_reg<2, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+    default: diag_reg<3, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+#else
+    default: diag_reg<2, CTRL>(v, tid, cm, rz, ph0, ph1); break;
+#endif
+    }
+}
+
+__device__ __forceinline__ void apply_diag(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
+    if (G.cm == 0)
+        apply_diag_t<false>(v, base, tid, G);
+    else
+        apply_diag_t<true>(v, base, tid, G);
+}
+
+// The tile number spread over the bits outside Q (a 0 inserted at each position of Q).
+__device__ __forceinline__ uint64_t tile_base(uint64_t k, const TileSpec &ts) {
+#pragma unroll
+    for (int j = 0; j < TL; ++j) {
+        const uint64_t lo = k & ts.qlo[j];
+        k = ((k ^ lo) << 1) | lo;
+    }
+    return k;
+}
+
+__device__ __forceinline__ uint64_t slot_offset(uint64_t dt, int r, const TileSpec &ts) {
+    uint64_t o = dt;
+#pragma unroll
+    for (int b = 0; b < TL - 8; ++b)
+        if ((r >> b) & 1) o |= ts.qbit[8 + b];
+    return o;
+}
+
+
+inline bool is_diagonal(int kind) { return kind == KZ || kind == KP || kind == KRZ; }
+
+// A gate with its physical target and controls, ready for a pass.
+struct PhysOp {
+    int kind;
+    uint32_t target;
+    uint64_t ctrl;  // physical control mask without the target bit
+    double angle;
+};
+
+struct Pass {
+    std::vector<PhysOp> ops;
+    std::vector<uint32_t> targets;  // distinct non-diagonal targets, in first-use order
+};
+
+// Tile positions for a pass: qubits 0..2 on lane bits 0..2; the most used
+// targets on register bits, then lane bits 3..5, then wave bits 6..7; free
+// positions take the lowest qubits not used yet.
+inline void assign_positions(uint32_t n, const Pass &p, int pos_of[64], uint32_t q_at[TL]) {
+    for (int i = 0; i < 64; ++i) pos_of[i] = -1;
+    for (int b = 0; b < TL; ++b) q_at[b] = 0xFFFFFFFFu;
+    for (uint32_t q = 0; q < 3; ++q) {
+        q_at[q] = q;
+        pos_of[q] = (int)q;
+    }
+    std::vector<std::pair<int, uint32_t>> freq;
+    for (uint32_t t : p.targets) {
+        if (t < 3) continue;
+        int f = 0;
+        for (const PhysOp &o : p.ops)
+            if (!is_diagonal(o.kind) && o.target == t) ++f;
+        freq.push_back({-f, t});
+    }
+    std::stable_sort(freq.begin(), freq.end(),
+                     [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
+                         return x.first < y.first;
+                     });
+    int order[TL - 3], fill_order[TL - 3];
+    {
+        int o = 0;
+        for (int b = 8; b < TL; ++b) order[o++] = b;  // register bits first
+        for (int b = 3; b < 8; ++b) order[o++] = b;   // then lane bits 3..5, wave bits 6..7
+        for (int b = 0; b < TL - 3; ++b) fill_order[b] = 3 + b;
+    }
+    size_t k = 0;
+    for (; k < freq.size(); ++k) {
+        q_at[order[k]] = freq[k].second;
+        pos_of[freq[k].second] = order[k];
+    }
+    // free positions take the lowest unused qubits (n >= TL leaves enough)
+    uint32_t filler = 3;
+    for (int i = 0; i < TL - 3; ++i) {
+        const int b = fill_order[i];
+        if (q_at[b] != 0xFFFFFFFFu) continue;
+        while (pos_of[filler] >= 0) ++filler;
+        q_at[b] = filler;
+        pos_of[filler] = b;
+        ++filler;
+    }
+}
+
+
+// The generic pair coefficients / diagonal phases of a gate (see pair_out).
+inline void set_coefficients(TGate &g, int kind, double angle) {
+    const Coef k = make_coef(kind, angle);
+    g.d = 1.0;
+    switch (kind) {
+    case KX: g.bu = 1.0; break;
+    case KY: g.crs = -1.0; g.cis = 1.0; break;
+    case KH: g.as = -1.0; g.bu = 1.0; g.d = 0.70710678118654752440; break;
+    case KRX: g.au = k.c; g.cru = k.s; g.ciu = -k.s; break;
+    case KRY: g.au = k.c; g.bs = k.s; break;
+    case KZ: g.ph0x = 1.0; g.ph1x = -1.0; g.ph1y = -0.0; break;  // dense_v2.rs:191
+    case KP: g.ph0x = 1.0; g.ph1x = k.ph1.x; g.ph1y = k.ph1.y; break;
+    default: g.ph0x = k.ph0.x; g.ph0y = k.ph0.y; g.ph1x = k.ph1.x; g.ph1y = k.ph1.y; break;
+    }
+}
+
+// The device record of one gate of a pass, relative to the pass's tile positions.
+inline TGate make_tgate(const PhysOp &o, const int pos_of[64], uint32_t n) {
+    TGate g;
+    memset(&g, 0, sizeof g);
+    g.kind = (uint32_t)o.kind;
+    g.tpos = pos_of[o.target] >= 0 ? (uint32_t)pos_of[o.target] : kOutside;
+    g.otb = 1ull << o.target;
+    for (uint32_t q = 0; q < n; ++q) {
+        if (!(o.ctrl >> q & 1ull)) continue;
+        if (pos_of[q] >= 0)
+            g.cm |= 1u << pos_of[q];
+        else
+            g.ocm |= 1ull << q;
+    }
+    set_coefficients(g, o.kind, o.angle);
+    return g;
+}
+
+inline TileSpec make_tile_spec(const uint32_t q_at[TL]) {
+    TileSpec ts;
+    uint64_t qmask = 0;
+    for (int b = 0; b < TL; ++b) {
+        ts.qbit[b] = 1ull << q_at[b];
+        qmask |= 1ull << q_at[b];
+    }
+    const Insert ins = make_insert(qmask);
+    for (int j = 0; j < TL; ++j) ts.qlo[j] = ins.mlo[j];
+    return ts;
+}
+
+// Greedy in-order pass building: a new non-diagonal target that does not fit
+// the TL - 3 free tile positions closes the pass.
+inline bool fits(const Pass &p, const PhysOp &o) {
+    if (is_diagonal(o.kind) || o.target < 3) return true;
+    if (std::find(p.targets.begin(), p.targets.end(), o.target) != p.targets.end()) return true;
+    return p.targets.size() < (size_t)(TL - 3);
+}
+
+inline void add_op(Pass &p, const PhysOp &o) {
+    if (!is_diagonal(o.kind) && o.target >= 3 &&
+        std::find(p.targets.begin(), p.targets.end(), o.target) == p.targets.end())
+        p.targets.push_back(o.target);
+    p.ops.push_back(o);
+}
+
+}  // namespace tile
+}  // namespace kbwh

@@ -1,0 +1,79 @@
+"""Batched persistent launch (kbwh_batch_*) against the oracle, the per-gate
+path, and itself under hipGraph replay."""
+import numpy as np
+import pytest
+
+import oracle
+from kbw_amd import DenseHip, circuits
+from kbw_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_expval(n, ops, params, ham):
+    o = oracle.DenseV2(n)
+    o.run(ops, params)
+    return o.exp_value(ham.pauli_strings, ham.coefficients), o.state()
+
+
+def mixed_batch(seed):
+    rng = np.random.default_rng(seed)
+    circs, params = [], []
+    for n in (3, 10, 11, 12, 14, 17, 9, 13):
+        if n % 2:
+            ops, p = circuits.hea(n, 3, seed=int(rng.integers(1 << 30)))
+        else:
+            ops = circuits.random_circuit(rng, n, 80, max_controls=2)
+            rot = np.isin(ops["kind"], [4, 5, 6, 7])
+            ops["param"][rot] = np.arange(rot.sum())
+            p = ops["angle"][rot].copy()
+        ham = circuits.random_hamiltonian(rng, n, 10, max_weight=3)
+        circs.append((n, ops, ham, len(p)))
+        params.append(p)
+    return circs, np.concatenate(params) if params else np.zeros(0)
+
+
+def test_batch_matches_oracle():
+    circs, params = mixed_batch(1)
+    b = Batch(circs)
+    out = b.run(params)
+    off = b.param_offsets
+    for i, (n, ops, ham, npc) in enumerate(circs):
+        e, st = oracle_expval(n, ops, params[off[i]:off[i + 1]], ham)
+        assert abs(out[i] - e) <= 1e-12 * max(1.0, abs(e)), (i, n, out[i], e)
+        got = b.state(i, 1 << n)
+        assert np.max(np.abs(got - st)) < 1e-13
+
+
+def test_batch_graph_replays_match_eager():
+    circs, params = mixed_batch(2)
+    b = Batch(circs)
+    rng = np.random.default_rng(3)
+    sets = [params, params + rng.normal(0, 0.3, size=params.shape), params * 0.5]
+    eager = [b.run(p) for p in sets]
+    b.use_graph(True)
+    for rep in range(3):
+        for k, p in enumerate(sets):
+            got = b.run(p)
+            assert np.array_equal(got, eager[k]), (rep, k)
+        b.use_graph(False)
+        assert np.array_equal(b.run(sets[1]), eager[1])  # eager work between replays
+        b.use_graph(True)
+
+
+def test_batch_c3_workload_spot_check():
+    circs, params = circuits.hea_batch(4096, seed=12)
+    b = Batch(circs)
+    out = b.run(params)
+    assert np.all(np.isfinite(out))
+    gates, passes, amps = b.stats()
+    print(f"c3: {len(circs)} circuits, {gates} gates, {passes} passes", flush=True)
+    off = b.param_offsets
+    rng = np.random.default_rng(4)
+    for i in rng.choice(len(circs), size=6, replace=False):
+        n, ops, ham, _ = circs[i]
+        s = DenseHip(n)
+        s.run(ops, params=params[off[i]:off[i + 1]])
+        e = s.exp_value(ham)
+        s.close()
+        assert abs(out[i] - e) <= 1e-12 * max(1.0, abs(e)), (i, n, out[i], e)

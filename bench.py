@@ -16,6 +16,12 @@ stream inside the timed region.
 cpu_baseline: the oracle (the C restatement of the reference's DenseV2 with
 OpenMP in place of rayon) on the same 30-qubit state and the same gates,
 on a bounded prefix of the circuit.
+
+Side legs, reported in their own objects: "exp_value" (64 Pauli strings on
+the 30-qubit state), "adjoint" (24-qubit, 576-parameter gradient against
+parameter shift on the plain path, plus 256 parameter sets at 16 qubits),
+"batch" (4096 ansatz circuits of 10-20 qubits in one persistent launch,
+circuits/s) and, on request, "block33" (the 33-qubit blocked QFT circuit).
 """
 import argparse
 import json
@@ -43,8 +49,8 @@ def parse():
     ap.add_argument("--layers", type=int, default=20)
     ap.add_argument("--mode", choices=["per_gate", "tiled", "best"], default="best")
     ap.add_argument("--cpu-gates", type=int, default=150, help="oracle sample size (gates); 0 = skip")
-    ap.add_argument("--legs", default="tiled,per_gate,exp,adjoint,cpu",
-                    help="comma list of legs: tiled, per_gate, exp, adjoint, cpu")
+    ap.add_argument("--legs", default="tiled,per_gate,exp,adjoint,batch,cpu",
+                    help="comma list of legs: tiled, per_gate, exp, adjoint, batch, cpu, c2")
     ap.add_argument("--pmc", action="store_true",
                     help="counter run: a 1 GiB torch copy then 4 pair passes (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -156,6 +162,83 @@ def adjoint_leg(subset=4, batch=256):
         out["batch16"] = {"sets": batch, "params_per_set": sets.shape[1], "s": t_b,
                           "grads_per_s": batch * sets.shape[1] / t_b}
     return out
+
+
+def batch_leg(num=4096, reps=3, single_sample=64, cpu_sample=96):
+    """4096 hardware-efficient-ansatz circuits (n = U{10..20}, 8 layers of
+    RY, RZ and a CZ ring, TFIM energy each) in one persistent launch per run:
+    circuits/s eager and hipGraph-replayed (new parameters every run), against
+    the same circuits run one by one through a DenseHip handle (tiled run +
+    exp_value) and through the oracle on the host cores, both on a seeded
+    sample of the circuits."""
+    import numpy as np
+
+    from kbw_amd import DenseHip, RUN_TILED, circuits
+    from kbw_amd.batch import Batch
+
+    circs, params = circuits.hea_batch(num, seed=12)
+    b = Batch(circs)
+    gates, passes, amp_passes = b.stats()
+    rng = np.random.default_rng(13)
+    sets = [params + rng.normal(0.0, 0.1, size=params.shape) for _ in range(reps)]
+    b.run(params)
+    t0 = time.perf_counter()
+    for p in sets:
+        b.run(p)
+    t_eager = (time.perf_counter() - t0) / reps
+    b.use_graph(True)
+    b.run(params)  # capture
+    t0 = time.perf_counter()
+    for p in sets:
+        out = b.run(p)
+    t_graph = (time.perf_counter() - t0) / reps
+    b.close()
+    off = np.concatenate([[0], np.cumsum([c[3] for c in circs])])
+    nq = np.array([c[0] for c in circs])
+    # the same work as separate handles, on a sample
+    pick = rng.choice(num, size=single_sample, replace=False)
+    t_single = 0.0
+    for i in pick:
+        n, ops, ham, _ = circs[i]
+        s = DenseHip(n)
+        t1 = time.perf_counter()
+        s.run(ops, params=sets[-1][off[i]:off[i + 1]], mode=RUN_TILED)
+        e = s.exp_value(ham)
+        t_single += time.perf_counter() - t1
+        s.close()
+        assert abs(e - out[i]) <= 1e-12 * max(1.0, abs(e)), (i, e, out[i])
+    single_rate = single_sample / t_single
+    res = {"workload": f"hea-batch-{num}-circuits-10to20q-8layers-tfim-seed12", "circuits": num,
+           "gates": gates, "passes": passes, "mean_qubits": float(nq.mean()),
+           "eager_s": t_eager, "graph_s": t_graph,
+           "circuits_per_s": num / min(t_eager, t_graph),
+           "circuits_per_s_eager": num / t_eager, "circuits_per_s_graph": num / t_graph,
+           "tile_pass_GBps": 32.0 * amp_passes / t_graph / 1e9,
+           "one_handle_per_circuit_circuits_per_s": single_rate,
+           "speedup_vs_one_handle_per_circuit": num / min(t_eager, t_graph) / single_rate,
+           "one_handle_sample": int(single_sample)}
+    if cpu_sample:
+        import oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpick = rng.choice(num, size=cpu_sample, replace=False)
+        t1 = time.perf_counter()
+        for i in cpick:
+            n, ops, ham, _ = circs[i]
+            o = oracle.DenseV2(n)
+            o.run(ops, sets[-1][off[i]:off[i + 1]])
+            o.exp_value(ham.pauli_strings, ham.coefficients)
+            del o
+        dt = time.perf_counter() - t1
+        # the sample's mean qubit count differs from the batch's: scale by amplitudes
+        amps_sample = float(np.sum(2.0 ** nq[cpick]))
+        amps_all = float(np.sum(2.0 ** nq))
+        res["cpu_baseline"] = {"value": num / (dt * amps_all / amps_sample), "unit": "circuits/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{cpu_sample} seeded circuits of the batch on the oracle "
+                                         f"(OpenMP {threads} threads), {dt:.2f} s, scaled to the batch "
+                                         f"by total amplitudes"}
+    return res
 
 
 def cpu_baseline(n, ops, ngates):
@@ -299,6 +382,8 @@ def main():
 
     if "adjoint" in legs:
         result["adjoint"] = adjoint_leg()
+    if "batch" in legs:
+        result["batch"] = batch_leg()
     if "c2" in legs:
         result["block33"] = block_leg()
     if args.cpu_gates > 0 and "cpu" in legs:

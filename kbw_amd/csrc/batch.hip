@@ -4,16 +4,22 @@
 // amplitudes; a circuit narrower than a tile runs with idle extra qubits that
 // stay |0>, which leaves its amplitudes unchanged).  Its gates are planned
 // once on the host into tiled passes (the same planner and gate application
-// as KBWH_RUN_TILED).  A run is three kernels:
+// as KBWH_RUN_TILED); its Hamiltonian is evaluated inside passes too:
+//   * every diagonal string, and every string whose X/Y qubits are tile
+//     qubits of the circuit's last gate pass, is summed from that pass's
+//     tiles while they are still in registers;
+//   * the other strings get read-only passes whose tiles are chosen to hold
+//     their X/Y qubits (greedily, several strings per pass).
+// A run is three kernels:
 //   k_coef   -- the coefficients of every parameterised gate from the
 //               parameter vector (device sincos), once per run;
 //   k_reset  -- the work counter back to 0 (a kernel, not a memset, so the
 //               run can be captured in a hipGraph);
 //   k_batch  -- persistent: every workgroup takes whole circuits from the
-//               counter, largest first, prepares |0...0>, applies each pass
-//               to every tile of its circuit, then reduces the circuit's
-//               Hamiltonian; a workgroup exits when the counter runs past the
-//               last circuit, so every wave reaches the exit.
+//               counter, largest first, and runs the circuit's passes over
+//               all of its tiles (the first pass makes |0...0> in registers
+//               instead of loading); a workgroup exits when the counter runs
+//               past the last circuit, so every wave reaches the exit.
 // With graphs enabled, the parameter upload, the three kernels and the result
 // download are captured once and replayed by later runs.
 #include <math.h>
@@ -23,6 +29,7 @@
 #include <numeric>
 #include <vector>
 
+#include "internal.h"
 #include "state.h"
 #include "tile_core.h"
 
@@ -34,23 +41,29 @@ using namespace kbwh::tile;
 namespace {
 
 constexpr uint32_t kMaxBatchQubits = 26;
+constexpr uint32_t kPassInit = 1;   // first pass: |0...0> instead of a load
+constexpr uint32_t kPassStore = 2;  // gate pass: write the tile back
+constexpr uint32_t kPassLds = 4;    // some string of the pass has X/Y qubits
 
 struct BCirc {
     uint64_t state_off;   // first amplitude of the circuit's slice
     uint32_t n_eff;       // max(n, TL)
     uint32_t first_pass, num_passes;
-    uint32_t first_string, num_strings;
-    uint32_t pad;
+    uint32_t num_strings;
+    uint32_t pad[2];
 };
 
 struct BPass {
     TileSpec ts;
     uint32_t first_gate, num_gates;
+    uint32_t first_str, num_str;
+    uint32_t flags, pad;
 };
 
+// P|j> = i^ny (-1)^popcount(j & z) |j ^ x>; xt is x as a mask of tile positions
 struct BString {
     uint64_t x, z;
-    uint32_t ny, pad;
+    uint32_t ny, xt;
     double coeff;
 };
 
@@ -113,11 +126,11 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     dv2 *__restrict__ states, const BCirc *__restrict__ circs, const uint32_t *__restrict__ order,
     uint32_t ncirc, const BPass *__restrict__ passes, const TGate *__restrict__ gates,
     const BString *__restrict__ strings, double *__restrict__ results, uint32_t *__restrict__ counter) {
-    __shared__ dv2 lds[TT * kChunk];
+    __shared__ dv2 lds[TT * TR];  // gate exchanges use the first TT * kChunk slots
     __shared__ double ws[TT / 64];
     __shared__ uint32_t s_item;
     const uint32_t tid = threadIdx.x;
-    // Thread 0 fetches the next item in ONE place per iteration (together
+    // Thread 0 fetches the next item in one place per iteration (together
     // with the result store) and every wave reads it back as a uniform value:
     // with a second `tid == 0` block at the loop top the compiler merged the
     // two into a loop nest in which lanes 1..63 re-entered the item read
@@ -131,13 +144,11 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
         const uint32_t ci = order[item];
         const BCirc C = circs[ci];
         dv2 *a = states + C.state_off;
-        const uint64_t size = 1ull << C.n_eff;
-        for (uint64_t i = tid; i < size; i += TT) a[i] = mk2(i == 0 ? 1.0 : 0.0, 0.0);
-        __threadfence_block();
-        __syncthreads();
         const uint32_t ntiles = 1u << (C.n_eff - TL);
+        double total = 0.0;
         for (uint32_t p = 0; p < C.num_passes; ++p) {
             const BPass &P = passes[C.first_pass + p];
+            const uint32_t flags = P.flags;
             uint64_t tmask = 0;
 #pragma unroll
             for (int b = 0; b < 8; ++b)
@@ -146,8 +157,13 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
                 const uint64_t base = tile_base(t, P.ts);
                 const uint64_t dt = base | tmask;
                 dv2 v[TR];
+                if (flags & kPassInit) {
 #pragma unroll
-                for (int r = 0; r < TR; ++r) v[r] = a[slot_offset(dt, r, P.ts)];
+                    for (int r = 0; r < TR; ++r) v[r] = mk2(slot_offset(dt, r, P.ts) == 0 ? 1.0 : 0.0, 0.0);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < TR; ++r) v[r] = a[slot_offset(dt, r, P.ts)];
+                }
                 for (uint32_t i = 0; i < P.num_gates; ++i) {
                     const TGate G = gates[P.first_gate + i];
                     if ((base & G.ocm) != G.ocm) continue;
@@ -156,36 +172,44 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
                     else
                         apply_pair(v, lds, tid, G);
                 }
+                if (flags & kPassStore) {
 #pragma unroll
-                for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, P.ts)] = v[r];
+                    for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, P.ts)] = v[r];
+                }
+                if (P.num_str) {
+                    // <H> = sum_k coeff_k <P_k>, <P> = sum_j conj(a_{j^x}) i^ny (-1)^{j.z} a_j
+                    if (flags & kPassLds) {
+                        __syncthreads();  // earlier exchange reads are done
+#pragma unroll
+                        for (int r = 0; r < TR; ++r) lds[tid | (r << 8)] = v[r];
+                        __syncthreads();
+                    }
+                    double acc = 0.0;
+                    for (uint32_t k = 0; k < P.num_str; ++k) {
+                        const BString S = strings[P.first_str + k];
+                        double sk = 0.0;
+#pragma unroll
+                        for (int r = 0; r < TR; ++r) {
+                            const bool neg = __popcll(slot_offset(dt, r, P.ts) & S.z) & 1;
+                            double term;
+                            if (S.xt == 0) {
+                                term = v[r].x * v[r].x + v[r].y * v[r].y;
+                            } else {
+                                const dv2 o = lds[(tid | (r << 8)) ^ S.xt], m = v[r];
+                                const double wr = o.x * m.x + o.y * m.y;
+                                const double wi = o.x * m.y - o.y * m.x;
+                                const uint32_t q = S.ny & 3u;
+                                term = (q == 0) ? wr : (q == 1) ? -wi : (q == 2) ? -wr : wi;
+                            }
+                            sk += neg ? -term : term;
+                        }
+                        acc += S.coeff * sk;
+                    }
+                    total += block_sum(acc, ws);
+                }
             }
             __threadfence_block();
             __syncthreads();
-        }
-        // <H> = sum_k coeff_k <P_k>, in string order (quantum_execution.rs:155-157)
-        double total = 0.0;
-        for (uint32_t k = 0; k < C.num_strings; ++k) {
-            const BString S = strings[C.first_string + k];
-            double acc = 0.0;
-            if (S.x == 0) {
-                for (uint64_t i = tid; i < size; i += TT) {
-                    const dv2 x = a[i];
-                    const double w = x.x * x.x + x.y * x.y;
-                    acc += (__popcll(i & S.z) & 1) ? -w : w;
-                }
-            } else {
-                const uint64_t hmlo = (1ull << __builtin_ctzll(S.x)) - 1;
-                for (uint64_t k2 = tid; k2 < (size >> 1); k2 += TT) {
-                    const uint64_t lo = k2 & hmlo;
-                    const uint64_t i = ((k2 ^ lo) << 1) | lo, j = i ^ S.x;
-                    const dv2 ai = a[i], aj = a[j];
-                    const double wr = aj.x * ai.x + aj.y * ai.y;
-                    const double wi = aj.x * ai.y - aj.y * ai.x;
-                    const uint32_t q = (S.ny + 2u * (uint32_t)(__popcll(i & S.z) & 1)) & 3u;
-                    acc += 2.0 * ((q == 0) ? wr : (q == 1) ? -wi : (q == 2) ? -wr : wi);
-                }
-            }
-            total += block_sum(acc, ws) * S.coeff;
         }
         if (tid == 0) {
             results[ci] = total;
@@ -310,6 +334,7 @@ int32_t build_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kb
             uint32_t q_at[TL];
             assign_positions(n_eff, cur, pos_of, q_at);
             BPass P;
+            memset(&P, 0, sizeof P);
             P.ts = make_tile_spec(q_at);
             P.first_gate = (uint32_t)tmpl.size();
             P.num_gates = (uint32_t)cur.ops.size();
@@ -339,11 +364,25 @@ int32_t build_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kb
             pidx.push_back(gp);
         }
         close();
-        B.num_passes = (uint32_t)passes.size() - B.first_pass;
+        const uint32_t gate_passes = (uint32_t)passes.size() - B.first_pass;
         // attach parameter indices to this circuit's gate records, in order
-        size_t gi = passes.empty() || B.num_passes == 0 ? tmpl.size() : passes[B.first_pass].first_gate;
-        for (size_t k = 0; k < pidx.size(); ++k) tmpl[gi + k].pad = (uint32_t)pidx[k];
-        B.first_string = (uint32_t)strings.size();
+        if (gate_passes) {
+            const size_t gi = passes[B.first_pass].first_gate;
+            for (size_t k = 0; k < pidx.size(); ++k) tmpl[gi + k].pad = (uint32_t)pidx[k];
+        } else {
+            // no gates: one empty pass prepares and stores |0...0>
+            uint32_t q_at[TL];
+            for (int b = 0; b < TL; ++b) q_at[b] = (uint32_t)b;
+            BPass P;
+            memset(&P, 0, sizeof P);
+            P.ts = make_tile_spec(q_at);
+            P.first_gate = (uint32_t)tmpl.size();
+            passes.push_back(P);
+        }
+        for (uint32_t p = B.first_pass; p < (uint32_t)passes.size(); ++p) passes[p].flags = kPassStore;
+        passes[B.first_pass].flags |= kPassInit;
+        // the Hamiltonian: decode the strings
+        std::vector<BString> hs;
         B.num_strings = K.num_strings;
         for (uint32_t k = K.first_string; k < K.first_string + K.num_strings; ++k) {
             if (!offsets || !coeffs) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null Hamiltonian arrays");
@@ -360,8 +399,52 @@ int32_t build_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kb
                 else if (paulis[j] == KBWH_PAULI_Z) S.z |= bit;
                 else return set_error(KBWH_ERR_INVALID_ARGUMENT, "unknown Pauli letter");
             }
-            strings.push_back(S);
+            hs.push_back(S);
         }
+        // strings evaluated in a pass: those whose X/Y qubits are tile qubits
+        auto take = [&](BPass &P, std::vector<bool> &done) {
+            uint64_t qmask = 0;
+            for (int b = 0; b < TL; ++b) qmask |= P.ts.qbit[b];
+            P.first_str = (uint32_t)strings.size();
+            for (size_t k = 0; k < hs.size(); ++k) {
+                if (done[k] || (hs[k].x & ~qmask)) continue;
+                BString S = hs[k];
+                S.xt = 0;
+                for (int b = 0; b < TL; ++b)
+                    if (S.x & P.ts.qbit[b]) S.xt |= 1u << b;
+                if (S.xt) P.flags |= kPassLds;
+                strings.push_back(S);
+                done[k] = true;
+            }
+            P.num_str = (uint32_t)strings.size() - P.first_str;
+        };
+        std::vector<bool> done(hs.size(), false);
+        take(passes.back(), done);  // the last gate pass, on the final state
+        for (;;) {
+            // a read-only pass for the rest: qubits 0..2, then the X/Y qubits of
+            // the remaining strings in order while they fit, then filler qubits
+            uint64_t qset = 7;
+            bool any = false;
+            for (size_t k = 0; k < hs.size(); ++k) {
+                if (done[k]) continue;
+                any = true;
+                if (__builtin_popcountll(qset | hs[k].x) <= TL) qset |= hs[k].x;
+            }
+            if (!any) break;
+            for (uint32_t q = 3; q < n_eff && __builtin_popcountll(qset) < TL; ++q) qset |= 1ull << q;
+            uint32_t q_at[TL];
+            int b = 0;
+            for (uint32_t q = 0; q < n_eff; ++q)
+                if (qset >> q & 1ull) q_at[b++] = q;
+            BPass P;
+            memset(&P, 0, sizeof P);
+            P.ts = make_tile_spec(q_at);
+            P.first_gate = (uint32_t)tmpl.size();
+            passes.push_back(P);
+            take(passes.back(), done);
+            if (passes.back().num_str == 0) return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: string placement");
+        }
+        B.num_passes = (uint32_t)passes.size() - B.first_pass;
     }
     // plan checks
     for (const BCirc &B : circs) {
@@ -375,6 +458,10 @@ int32_t build_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kb
                     return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: tile qubits");
                 seen |= q;
             }
+            if (P.first_str + P.num_str > strings.size()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: strings");
+            for (uint32_t k = P.first_str; k < P.first_str + P.num_str; ++k)
+                if (strings[k].xt >> TL || __builtin_popcountll(strings[k].x) != __builtin_popcount(strings[k].xt))
+                    return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: string outside tile");
             for (uint32_t i = P.first_gate; i < P.first_gate + P.num_gates; ++i) {
                 const TGate &g = tmpl[i];
                 const bool diag = g.kind == KZ || g.kind == KP || g.kind == KRZ;

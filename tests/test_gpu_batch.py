@@ -45,6 +45,34 @@ def test_batch_matches_oracle():
         assert np.max(np.abs(got - st)) < 1e-13
 
 
+def test_batch_string_placement_matches_oracle():
+    # strings evaluated in the last gate pass, in read-only passes, with Y
+    # factors, on gate-free circuits and on circuits narrower than a tile
+    rng = np.random.default_rng(21)
+    circs, params = [], []
+    empty = circuits.OpList().array()
+    ham_y = circuits.Hamiltonian([[("Y", 0)], [("X", 1), ("Y", 2)], [("Z", 0), ("Z", 2)]], [0.7, -1.1, 0.4])
+    circs.append((3, empty, ham_y, 0))
+    params.append(np.zeros(0))
+    for n in (20, 12, 19):
+        ops, p = circuits.hea(n, 2, seed=int(rng.integers(1 << 30)))
+        strings = [[("X", q)] for q in range(n)] + [[("Y", q), ("Y", (q + 7) % n)] for q in range(0, n, 3)]
+        strings += [[("Z", q), ("Z", q + 1)] for q in range(n - 1)]
+        ham = circuits.Hamiltonian(strings, rng.normal(0, 1, size=len(strings)))
+        circs.append((n, ops, ham, len(p)))
+        params.append(p)
+    circs.append((14, empty, circuits.Hamiltonian([[("X", 13)], [("Z", 0)]], [1.0, 2.0]), 0))
+    params.append(np.zeros(0))
+    params = np.concatenate(params)
+    b = Batch(circs)
+    out = b.run(params)
+    off = b.param_offsets
+    for i, (n, ops, ham, npc) in enumerate(circs):
+        e, st = oracle_expval(n, ops, params[off[i]:off[i + 1]], ham)
+        assert abs(out[i] - e) <= 1e-12 * max(1.0, abs(e)), (i, n, out[i], e)
+        assert np.max(np.abs(b.state(i, 1 << n) - st)) < 1e-13
+
+
 def test_batch_graph_replays_match_eager():
     circs, params = mixed_batch(2)
     b = Batch(circs)

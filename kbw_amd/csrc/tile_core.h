@@ -108,28 +108,59 @@ __device__ __forceinline__ dv2 pair_out(dv2 m, dv2 o, const PairCoef &k) {
     return mk2(((k.A * m.x + k.B * o.x) + k.Cr * o.y) * k.D, ((k.A * m.y + k.B * o.y) + k.Ci * o.x) * k.D);
 }
 
+// Uncontrolled Y, H, RX and RY in two terms instead of three:
+//     out = (A m + B q) D,   q = o or q = (o.im, -o.re)  (rotated: Y, RX)
+// with A, D as in the generic form and B = Cr for the rotated kinds:
+//     Y:  A=0    B=-sigma  rotated       (|0>: (o.im, -o.re), |1>: (-o.im, o.re))
+//     H:  A=-sigma B=1     D=1/sqrt2
+//     RX: A=c    B=s       rotated       (c m.re + s o.im, c m.im - s o.re)
+//     RY: A=c    B=sigma*s
+// 8 operations per amplitude instead of 12, each output equal in value to
+// the generic form's.  The kernels keep exactly two uncontrolled pair code
+// paths (X and this one): gfx950 code built with a separate path per kind
+// gave wrong amplitudes for every kind, X included.
+enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2 };
+
+// Control test for the amplitude in register slot r of thread tid, whose tile
+// index is tid + 256 r: the thread bits are tested once per gate (tok) and the
+// slot bits against the uniform part of cm, so no per-slot index is kept.
+__device__ __forceinline__ bool thread_ctrl_ok(uint32_t tid, uint32_t cm) { return (tid & (cm & 0xFFu)) == (cm & 0xFFu); }
+__device__ __forceinline__ bool slot_ctrl_ok(bool tok, int r, uint32_t cm) {
+    const uint32_t hi = cm >> 8;
+    return tok && (((uint32_t)r & hi) == hi);
+}
+
+template <int PK>
+__device__ __forceinline__ dv2 pair_val(dv2 m, dv2 o, bool one, const TGate &G) {
+    if constexpr (PK == PK_X) {
+        return o;
+    } else if constexpr (PK == PK_TWO) {
+        const PairCoef k = pair_coef(G, one ? 1.0 : -1.0);
+        const bool rot = G.kind == KRX || G.kind == KY;
+        const double b = rot ? k.Cr : k.B;
+        const double qx = rot ? o.y : o.x, qy = rot ? -o.x : o.y;
+        return mk2((k.A * m.x + b * qx) * k.D, (k.A * m.y + b * qy) * k.D);
+    } else {
+        return pair_out(m, o, pair_coef(G, one ? 1.0 : -1.0));
+    }
+}
+
 // Target on register bit RB: both amplitudes of a pair live in one thread.
-// SWAP: the X gate (dense_v2.rs:168-170), a pure exchange.  CTRL: some
-// control lies inside the tile, so only the pairs whose index matches cm change.
-template <int RB, bool SWAP, bool CTRL>
-__device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, uint32_t cm, const PairCoef &k0,
-                                         const PairCoef &k1) {
+// CTRL: some control lies inside the tile, so only the pairs whose index
+// matches cm change.
+template <int RB, int PK, bool CTRL>
+__device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, const TGate &G) {
+    const uint32_t cm = G.cm;
+    const bool tok = thread_ctrl_ok(tid, cm);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
         if ((r >> RB) & 1) continue;
         const int r1 = r | (1 << RB);
         const dv2 t0 = v[r], t1 = v[r1];
-        dv2 o0, o1;
-        if constexpr (SWAP) {
-            o0 = t1;
-            o1 = t0;
-        } else {
-            o0 = pair_out(t0, t1, k0);
-            o1 = pair_out(t1, t0, k1);
-        }
+        const dv2 o0 = pair_val<PK>(t0, t1, false, G);
+        const dv2 o1 = pair_val<PK>(t1, t0, true, G);
         if constexpr (CTRL) {
-            const uint32_t j0 = tid | ((uint32_t)r << 8);
-            const bool ok = (j0 & cm) == cm;
+            const bool ok = slot_ctrl_ok(tok, r, cm);
             v[r] = ok ? o0 : t0;
             v[r1] = ok ? o1 : t1;
         } else {
@@ -139,18 +170,20 @@ __device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, uint32_t cm
     }
 }
 
-template <bool SWAP, bool CTRL>
+template <int PK, bool CTRL>
 __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, uint32_t tp,
-                                            uint32_t cm, const PairCoef &k) {
+                                            const TGate &G) {
+    const uint32_t cm = G.cm;
+    const bool tok = thread_ctrl_ok(tid, cm);
+    const bool one = ((tid >> tp) & 1u) != 0;
     if (tp < 6) {
         const int lm = 1 << tp;
 #pragma unroll
         for (int r = 0; r < TR; ++r) {
             const dv2 o = shfl_xor2(v[r], lm);
-            const dv2 out = SWAP ? o : pair_out(v[r], o, k);
+            const dv2 out = pair_val<PK>(v[r], o, one, G);
             if constexpr (CTRL) {
-                const uint32_t j = tid | ((uint32_t)r << 8);
-                v[r] = ((j & cm) == cm) ? out : v[r];
+                v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
             } else {
                 v[r] = out;
             }
@@ -163,15 +196,14 @@ __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid
     for (int q = 0; q < TR; q += kChunk) {
         __syncthreads();  // earlier exchange reads are done
 #pragma unroll
-        for (int r = 0; r < kChunk; ++r) lds[tid | (r << 8)] = v[q + r];
+        for (int r = 0; r < kChunk; ++r) lds[tid + (r << 8)] = v[q + r];
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < kChunk; ++r) {
-            const dv2 o = lds[pt | (r << 8)];
-            const dv2 out = SWAP ? o : pair_out(v[q + r], o, k);
+            const dv2 o = lds[pt + (r << 8)];
+            const dv2 out = pair_val<PK>(v[q + r], o, one, G);
             if constexpr (CTRL) {
-                const uint32_t j = tid | ((uint32_t)(q + r) << 8);
-                v[q + r] = ((j & cm) == cm) ? out : v[q + r];
+                v[q + r] = slot_ctrl_ok(tok, q + r, cm) ? out : v[q + r];
             } else {
                 v[q + r] = out;
             }
@@ -179,105 +211,118 @@ __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid
     }
 }
 
-template <bool SWAP, bool CTRL>
+template <int PK, bool CTRL>
 __device__ __forceinline__ void apply_pair_t(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
-    const uint32_t tp = G.tpos, cm = G.cm;
+    const uint32_t tp = G.tpos;
     if (tp >= 8) {
-        const PairCoef k0 = pair_coef(G, -1.0), k1 = pair_coef(G, 1.0);
-        switch (tp) {
-        case 8: reg_pair<0, SWAP, CTRL>(v, tid, cm, k0, k1); break;
-        case 9: reg_pair<1, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+        // Register bit rb = tp - 8, dispatched on its bits.  A `switch (tp)`
+        // here was lowered by the gfx950 backend into a compare tree whose
+        // default case had two predecessors, and the structurized code never
+        // ran that case (the last register bit's gates were dropped).
+        const uint32_t rb = tp - 8;
+        if (rb & 2u) {
 #if KBWH_TILE_BITS > 11
-        case 10: reg_pair<2, SWAP, CTRL>(v, tid, cm, k0, k1); break;
-        default: reg_pair<3, SWAP, CTRL>(v, tid, cm, k0, k1); break;
-#else
-        default: reg_pair<2, SWAP, CTRL>(v, tid, cm, k0, k1); break;
+            if (rb & 1u)
+                reg_pair<3, PK, CTRL>(v, tid, G);
+            else
 #endif
+                reg_pair<2, PK, CTRL>(v, tid, G);
+        } else if (rb & 1u) {
+            reg_pair<1, PK, CTRL>(v, tid, G);
+        } else {
+            reg_pair<0, PK, CTRL>(v, tid, G);
         }
         return;
     }
-    const PairCoef k = pair_coef(G, ((tid >> tp) & 1u) ? 1.0 : -1.0);
-    thread_pair<SWAP, CTRL>(v, lds, tid, tp, cm, k);
+    thread_pair<PK, CTRL>(v, lds, tid, tp, G);
 }
 
 __device__ __forceinline__ void apply_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
-    const bool swap = G.kind == KX;
-    if (G.cm == 0) {
-        if (swap)
-            apply_pair_t<true, false>(v, lds, tid, G);
+    if (G.kind == KX) {
+        if (G.cm == 0)
+            apply_pair_t<PK_X, false>(v, lds, tid, G);
         else
-            apply_pair_t<false, false>(v, lds, tid, G);
-    } else {
-        if (swap)
-            apply_pair_t<true, true>(v, lds, tid, G);
-        else
-            apply_pair_t<false, true>(v, lds, tid, G);
+            apply_pair_t<PK_X, true>(v, lds, tid, G);
+        return;
     }
+    if (G.cm != 0)
+        apply_pair_t<PK_GEN, true>(v, lds, tid, G);
+    else
+        apply_pair_t<PK_TWO, false>(v, lds, tid, G);
 }
 
 // Z, P and RZ: each amplitude is multiplied by the phase of its half
 // (dense_v2.rs:190-191, 215-216, 271-272); Z and P leave the |0> half alone.
-template <bool CTRL>
+// NEG: the Z gate, whose product by -1 is the negation in value.
+template <bool NEG>
+__device__ __forceinline__ dv2 diag_val(dv2 x, dv2 ph) {
+    if constexpr (NEG)
+        return -x;
+    else
+        return cmul(x, ph);
+}
+
+template <bool NEG, bool CTRL>
 __device__ __forceinline__ void diag_all(dv2 (&v)[TR], uint32_t tid, uint32_t cm, dv2 ph) {
+    const bool tok = thread_ctrl_ok(tid, cm);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
-        const dv2 out = cmul(v[r], ph);
+        const dv2 out = diag_val<NEG>(v[r], ph);
         if constexpr (CTRL) {
-            const uint32_t j = tid | ((uint32_t)r << 8);
-            v[r] = ((j & cm) == cm) ? out : v[r];
+            v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
         } else {
             v[r] = out;
         }
     }
 }
 
-// target on register bit RB: which slots are |1> is known at compile time
-template <int RB, bool CTRL>
-__device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm, bool rz, dv2 ph0,
-                                         dv2 ph1) {
+// target on register bit rb: slot r is in the |1> half when bit rb of r is
+// set, a uniform test per slot (no switch over rb, whose near-identical cases
+// the compiler would merge into code that addresses v[] through pointers)
+template <bool NEG, bool CTRL>
+__device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm, uint32_t rb, bool rz,
+                                         dv2 ph0, dv2 ph1) {
+    const bool tok = thread_ctrl_ok(tid, cm);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
-        const bool one = (r >> RB) & 1;
+        const bool one = ((uint32_t)r >> rb) & 1u;
         if (!one && !rz) continue;
-        const dv2 out = cmul(v[r], one ? ph1 : ph0);
+        const dv2 out = diag_val<NEG>(v[r], one ? ph1 : ph0);
         if constexpr (CTRL) {
-            const uint32_t j = tid | ((uint32_t)r << 8);
-            v[r] = ((j & cm) == cm) ? out : v[r];
+            v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
         } else {
             v[r] = out;
         }
     }
 }
 
-template <bool CTRL>
+template <bool NEG, bool CTRL>
 __device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
     const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
     const uint32_t cm = G.cm, tp = G.tpos;
     const bool rz = G.kind == KRZ;
-    if (tp == kOutside || tp < 8) {
-        // the target bit is fixed per tile (outside Q) or per thread (lane/wave bit)
-        const bool one = tp == kOutside ? (base & G.otb) != 0 : ((tid >> tp) & 1u) != 0;
-        if (!one && !rz) return;
-        diag_all<CTRL>(v, tid, cm, one ? ph1 : ph0);
+    // one range test and a select (no multi-way branch on tp; see apply_pair_t)
+    if (tp - 8u < 8u) {
+        diag_reg<NEG, CTRL>(v, tid, cm, tp - 8u, rz, ph0, ph1);
         return;
     }
-    switch (tp) {
-    case 8: diag_reg<0, CTRL>(v, tid, cm, rz, ph0, ph1); break;
-    case 9: diag_reg<1, CTRL>(v, tid, cm, rz, ph0, ph1); break;
-#if KBWH_TILE_BITS > 11
-    case 10: diag_reg<2, CTRL>(v, tid, cm, rz, ph0, ph1); break;
-    default: diag_reg<3, CTRL>(v, tid, cm, rz, ph0, ph1); break;
-#else
-    default: diag_reg<2, CTRL>(v, tid, cm, rz, ph0, ph1); break;
-#endif
-    }
+    // the target bit is fixed per tile (outside Q) or per thread (lane/wave bit)
+    const bool one = tp == kOutside ? (base & G.otb) != 0 : ((tid >> (tp & 31u)) & 1u) != 0;
+    if (!one && !rz) return;
+    diag_all<NEG, CTRL>(v, tid, cm, one ? ph1 : ph0);
 }
 
 __device__ __forceinline__ void apply_diag(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
-    if (G.cm == 0)
-        apply_diag_t<false>(v, base, tid, G);
-    else
-        apply_diag_t<true>(v, base, tid, G);
+    if (G.kind == KZ) {
+        if (G.cm == 0)
+            apply_diag_t<true, false>(v, base, tid, G);
+        else
+            apply_diag_t<true, true>(v, base, tid, G);
+    } else if (G.cm == 0) {
+        apply_diag_t<false, false>(v, base, tid, G);
+    } else {
+        apply_diag_t<false, true>(v, base, tid, G);
+    }
 }
 
 // The tile number spread over the bits outside Q (a 0 inserted at each position of Q).

@@ -50,16 +50,24 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     dv2 v[TR];
 #pragma unroll
     for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
-    for (uint32_t i = 0; i < ng; ++i) {
+    // ng >= 1 (the host launches only non-empty passes); a do-while leaves no
+    // path around the loop, so the loaded values need no copy kept for one
+    uint32_t i = 0;
+    do {
         const TGate G = gates[i];
-        if ((base & G.ocm) != G.ocm) continue;  // a control outside the tile is 0 here
-        if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
-            apply_diag(v, base, tid, G);
-        else
-            apply_pair(v, lds, tid, G);
-    }
+        if ((base & G.ocm) == G.ocm) {  // a control outside the tile is 0 here
+            if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
+                apply_diag(v, base, tid, G);
+            else
+                apply_pair(v, lds, tid, G);
+        }
+    } while (++i < ng);
+    // recompute the store addresses instead of keeping eight of them live
+    // through the gate loop
+    uint64_t ds = dt;
+    asm volatile("" : "+v"(ds));
 #pragma unroll
-    for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(dt, r, ts));
+    for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, ts));
 }
 
 

@@ -42,6 +42,47 @@ def test_tiled_brickwork_bit_exact(n, layers):
     assert_same(g.state(), o.state())
 
 
+@pytest.mark.parametrize("kind", range(8))
+def test_tiled_every_kind_at_every_tile_position(kind):
+    """One gate of `kind` at each tile position: lane bits 0..5, wave bits 6..7,
+    register bits 8..10; uncontrolled, with a control on a lane bit, with one
+    on a register bit and with one outside the tile, on a random state.  In a
+    pass whose first-used targets are qubits 3..10 (each used once), the
+    planner gives them positions 8, 9, 10, 3, 4, 5, 6, 7 in that order; qubits
+    0..2 are always positions 0..2.  The other targets get X gates."""
+    print(f"tiled kind {kind} at every position", flush=True)
+    n = 13
+    rng = np.random.default_rng(40 + kind)
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    st /= np.linalg.norm(st)
+    cases = []
+    for test_q in range(11):
+        lane_c = [0] if test_q != 0 else [1]
+        reg_c = [3] if test_q != 3 else [4]
+        for ctrl in ([], lane_c, reg_c, [12]):
+            ops = circuits.OpList()
+            if test_q < 3:
+                ops.add(kind, test_q, ctrl, angle=0.9)
+            for t in range(3, 11):
+                if t == test_q:
+                    ops.add(kind, t, ctrl, angle=0.9)
+                else:
+                    ops.add(0, t)
+            cases.append((test_q, ctrl, ops.array()))
+    for test_q, ctrl, ops in cases:
+        a = DenseHip(n)
+        a.set_state(st)
+        a.run(ops, mode=RUN_TILED)
+        assert a.run_stats()[1] == 1
+        b = DenseHip(n)
+        b.set_state(st)
+        b.run(ops, mode=RUN_PER_GATE)
+        bad = np.nonzero(a.state() != b.state())[0]
+        assert bad.size == 0, (kind, test_q, ctrl, bad.size)
+        a.close()
+        b.close()
+
+
 def test_tiled_small_state_uses_per_gate():
     n = 8
     ops = circuits.random_circuit(np.random.default_rng(3), n, 100)

@@ -62,6 +62,8 @@ extern "C" {
 /* kbwh_run modes */
 #define KBWH_RUN_PER_GATE 0u /* one HBM pass per gate: the plain path */
 #define KBWH_RUN_TILED 1u    /* gates grouped into multi-gate passes over LDS tiles; same per-pair arithmetic, same result bits */
+#define KBWH_RUN_FUSED 2u    /* gates fused into k-qubit blocks applied with FP64 MFMA; equal to rounding (~1e-16 per block), not bit for bit */
+#define KBWH_RUN_AUTO 3u     /* the configuration chosen by kbwh_autotune for this handle (KBWH_RUN_TILED until tuned) */
 
 /* One circuit operation (24 bytes).  `angle` is used unless `param` >= 0
  * and a parameter vector is supplied, in which case the angle is
@@ -101,6 +103,29 @@ int32_t kbwh_gate_mask(kbwh_state *s, int32_t kind, double angle, uint32_t targe
  * (quantum_execution.rs:310-314, 374-377).  `params` may be NULL. */
 int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
                  uint32_t mode);
+
+/* Run configuration (no counterpart in the reference).  fusion_width: qubits
+ * per fused block for KBWH_RUN_FUSED, 4 or 5 (default 5).  tile_targets:
+ * most non-diagonal targets outside qubits 0..2 per tiled pass, 1..8
+ * (default 8, the tile's free positions). */
+int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_targets);
+
+/* Autotuner (no counterpart in the reference): times each candidate
+ * configuration of `ops` (at most the first 256 gates) on a scratch buffer
+ * the size of the state, leaving the state untouched, and keeps the fastest
+ * for KBWH_RUN_AUTO.  Candidates: per-gate; tiled with 8, 6 and 4 targets
+ * per pass; with allow_inexact, fused blocks of 4 and 5 qubits.  ms[i] is the
+ * time of candidate i in that order (0 when not run).  Needs device memory
+ * for a second state (KBWH_ERR_OUT_OF_MEMORY otherwise). */
+typedef struct kbwh_tune_result {
+    uint32_t mode;          /* KBWH_RUN_PER_GATE, KBWH_RUN_TILED or KBWH_RUN_FUSED */
+    uint32_t fusion_width;  /* for KBWH_RUN_FUSED */
+    uint32_t tile_targets;  /* for KBWH_RUN_TILED */
+    uint32_t num_candidates;
+    double ms[8];
+} kbwh_tune_result;
+int32_t kbwh_autotune(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                      int32_t allow_inexact, kbwh_tune_result *out);
 
 /* ---- measurement ----------------------------------------------------- */
 /* Replaces measure_p1 (dense_v2.rs:279-294). */

@@ -23,7 +23,7 @@ X, Y, Z, H, P, RX, RY, RZ = range(8)
 GATE_NAMES = {"x": X, "y": Y, "z": Z, "h": H, "p": P, "rx": RX, "ry": RY, "rz": RZ}
 PAULI_X, PAULI_Y, PAULI_Z = 1, 2, 3
 FLAG_BLOCK = 1
-RUN_PER_GATE, RUN_TILED = 0, 1
+RUN_PER_GATE, RUN_TILED, RUN_FUSED, RUN_AUTO = 0, 1, 2, 3
 
 # kbwh_op, 24 bytes
 OP_DTYPE = np.dtype([("kind", "u1"), ("target", "u1"), ("reserved", "<u2"), ("param", "<i4"),
@@ -34,6 +34,11 @@ assert OP_DTYPE.itemsize == 24
 CIRCUIT_DTYPE = np.dtype([("num_qubits", "<u4"), ("num_strings", "<u4"), ("first_op", "<u8"),
                           ("num_ops", "<u8"), ("first_string", "<u4"), ("first_param", "<u4")], align=True)
 assert CIRCUIT_DTYPE.itemsize == 32
+
+# kbwh_tune_result, 80 bytes
+TUNE_DTYPE = np.dtype([("mode", "<u4"), ("fusion_width", "<u4"), ("tile_targets", "<u4"),
+                       ("num_candidates", "<u4"), ("ms", "<f8", (8,))], align=True)
+assert TUNE_DTYPE.itemsize == 80
 
 # Every exported symbol: (name, argtypes).  All return int32 except kbwh_last_error.
 _vp, _u32, _u64, _i32, _f64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_double
@@ -59,6 +64,8 @@ SIGNATURES = {
     "kbwh_swap_count": [_vp, C.POINTER(_u64)],
     "kbwh_synchronize": [_vp],
     "kbwh_run_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64)],
+    "kbwh_set_run_config": [_vp, _u32, _u32],
+    "kbwh_autotune": [_vp, _p, _u64, _p, _i32, _p],
     "kbwh_stream": [_vp, C.POINTER(_vp)],
     "kbwh_device_buffer": [_vp, C.POINTER(_vp), C.POINTER(_u64)],
     "kbwh_batch_new": [_p, _u32, _p, _p, _p, _p, _p, _u64, C.POINTER(_vp)],

@@ -36,6 +36,10 @@ struct kbwh_state {
     void *tile_gates = nullptr;
     size_t tile_gates_cap = 0;
     uint64_t last_passes = 0;    // HBM passes of the last kbwh_run
+    // run configuration (kbwh_set_run_config / kbwh_autotune)
+    uint32_t fusion_width = 5;   // qubits per fused block
+    uint32_t tile_targets = 8;   // non-diagonal targets above qubit 2 per tiled pass
+    uint32_t auto_mode = KBWH_RUN_TILED;
     uint64_t last_gates = 0;     // gates of the last kbwh_run
     // adjoint differentiation: two extra state vectors, allocated on first use
     double2 *adj_phi = nullptr;
@@ -58,6 +62,8 @@ int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const dou
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
                      const double *params);
 void free_adjoint(kbwh_state *s);
+int32_t run_fused(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                  int k);
 
 // Pauli strings flattened to masks: x = X|Y qubits, z = Z|Y qubits, ny = #Y.
 struct PauliMasks {

@@ -269,7 +269,9 @@ int32_t kbwh_gate_mask(kbwh_state *s, int32_t kind, double angle, uint32_t targe
 int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
                  uint32_t mode) {
     if (!s || (num_ops && !ops)) return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    if (mode == KBWH_RUN_AUTO) mode = s->auto_mode;
     if (mode == KBWH_RUN_TILED) return run_tiled(s, ops, num_ops, params);
+    if (mode == KBWH_RUN_FUSED) return run_fused(s, s->amp, ops, num_ops, params, (int)s->fusion_width);
     if (mode != KBWH_RUN_PER_GATE) return fail(KBWH_ERR_INVALID_ARGUMENT, "unknown run mode");
     s->last_passes = num_ops;
     s->last_gates = num_ops;

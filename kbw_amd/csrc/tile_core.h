@@ -453,11 +453,11 @@ inline TileSpec make_tile_spec(const uint32_t q_at[TL]) {
 }
 
 // Greedy in-order pass building: a new non-diagonal target that does not fit
-// the TL - 3 free tile positions closes the pass.
-inline bool fits(const Pass &p, const PhysOp &o) {
+// the free tile positions (at most `cap` of the TL - 3) closes the pass.
+inline bool fits(const Pass &p, const PhysOp &o, uint32_t cap = TL - 3) {
     if (is_diagonal(o.kind) || o.target < 3) return true;
     if (std::find(p.targets.begin(), p.targets.end(), o.target) != p.targets.end()) return true;
-    return p.targets.size() < (size_t)(TL - 3);
+    return p.targets.size() < (size_t)std::min<uint32_t>(cap, TL - 3);
 }
 
 inline void add_op(Pass &p, const PhysOp &o) {

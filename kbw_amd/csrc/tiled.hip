@@ -164,7 +164,7 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         if (rc) return rc;
         uint64_t m = physical_mask(s, op.ctrl_mask) & ~(1ull << t);
         const PhysOp po{(int)op.kind, t, m, angle};
-        if (!fits(cur, po) && (rc = flush())) return rc;
+        if (!fits(cur, po, s->tile_targets) && (rc = flush())) return rc;
         add_op(cur, po);
     }
     int32_t rc = flush();

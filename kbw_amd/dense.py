@@ -145,6 +145,20 @@ class DenseHip:
         p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
         check(self._lib.kbwh_run(self._h, ptr(a), len(a), ptr(p), mode))
 
+    def set_run_config(self, fusion_width=5, tile_targets=8):
+        """Qubits per fused block (4 or 5) and non-diagonal targets per tiled pass (1..8)."""
+        check(self._lib.kbwh_set_run_config(self._h, fusion_width, tile_targets))
+
+    def autotune(self, ops, params=None, allow_inexact=False):
+        """Time the run configurations on a scratch buffer and keep the fastest
+        for RUN_AUTO; returns (mode, fusion_width, tile_targets, ms per candidate)."""
+        a = ops_array(ops)
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        out = np.zeros(1, dtype=_lib.TUNE_DTYPE)
+        check(self._lib.kbwh_autotune(self._h, ptr(a), len(a), ptr(p), 1 if allow_inexact else 0, ptr(out)))
+        r = out[0]
+        return int(r["mode"]), int(r["fusion_width"]), int(r["tile_targets"]), r["ms"][: int(r["num_candidates"])].copy()
+
     # ---- measurement ----------------------------------------------------
     def measure_p1(self, target):
         out = C.c_double()

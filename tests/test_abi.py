@@ -43,12 +43,20 @@ def test_constants_match_header():
     assert int(consts["KBWH_ERR_EXECUTION_FAILED"]) == _lib.ERR_EXECUTION_FAILED
     assert int(consts["KBWH_PAULI_Y"]) == _lib.PAULI_Y
     assert int(consts["KBWH_RUN_TILED"]) == _lib.RUN_TILED
+    assert int(consts["KBWH_RUN_FUSED"]) == _lib.RUN_FUSED and int(consts["KBWH_RUN_AUTO"]) == _lib.RUN_AUTO
+
+
+def test_tune_result_layout_matches_header():
+    assert _lib.TUNE_DTYPE.itemsize == 80
+    assert [_lib.TUNE_DTYPE.fields[f][1] for f in ("mode", "fusion_width", "tile_targets", "num_candidates", "ms")] == [0, 4, 8, 12, 16]
 
 
 def test_invalid_arguments_rejected_without_device():
     lib = kbw_amd.load()
     assert lib.kbwh_new(4, 0, 0, None) == _lib.ERR_INVALID_ARGUMENT
     assert lib.kbwh_gate_mask(None, 0, 0.0, 0, 0) == _lib.ERR_INVALID_ARGUMENT
+    assert lib.kbwh_set_run_config(None, 5, 8) == _lib.ERR_INVALID_ARGUMENT
+    assert lib.kbwh_autotune(None, None, 0, None, 0, None) == _lib.ERR_INVALID_ARGUMENT
     assert b"null" in lib.kbwh_last_error()
 
 

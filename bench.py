@@ -21,7 +21,9 @@ Side legs, reported in their own objects: "exp_value" (64 Pauli strings on
 the 30-qubit state), "adjoint" (24-qubit, 576-parameter gradient against
 parameter shift on the plain path, plus 256 parameter sets at 16 qubits),
 "batch" (4096 ansatz circuits of 10-20 qubits in one persistent launch,
-circuits/s) and, on request, "block33" (the 33-qubit blocked QFT circuit).
+circuits/s), "live12" (the 12-qubit Grover circuit given gate by gate:
+one launch per gate against queued gates in one launch) and, on request,
+"block33" (the 33-qubit blocked QFT circuit).
 """
 import argparse
 import json
@@ -49,8 +51,8 @@ def parse():
     ap.add_argument("--layers", type=int, default=20)
     ap.add_argument("--mode", choices=["per_gate", "tiled", "best"], default="best")
     ap.add_argument("--cpu-gates", type=int, default=150, help="oracle sample size (gates); 0 = skip")
-    ap.add_argument("--legs", default="tiled,per_gate,exp,adjoint,batch,cpu",
-                    help="comma list of legs: tiled, per_gate, exp, adjoint, batch, cpu, c2")
+    ap.add_argument("--legs", default="tiled,per_gate,exp,adjoint,batch,live,cpu",
+                    help="comma list of legs: tiled, per_gate, exp, adjoint, batch, live, cpu, c2")
     ap.add_argument("--pmc", action="store_true",
                     help="counter run: a 1 GiB torch copy then 4 pair passes (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -241,6 +243,50 @@ def batch_leg(num=4096, reps=3, single_sample=64, cpu_sample=96):
     return res
 
 
+def live_leg(reps=5):
+    """12-qubit Grover search (50 iterations, seed 0) given gate by gate, as a
+    live QubitManager does, then a full measurement: one launch per gate
+    against the queued path (the whole circuit in one launch), with the oracle
+    on the host for the same gates."""
+    import numpy as np
+
+    import oracle
+    from kbw_amd import DenseHip, circuits
+
+    n = 12
+    ops, target = circuits.grover(n, 50, seed=0)
+    rows = [(int(o["kind"]), float(o["angle"]), int(o["target"]), int(o["ctrl_mask"])) for o in ops]
+    u = np.random.default_rng(1).random(n)
+    out = {"workload": "grover-12q-50iter-seed0", "gates": len(rows)}
+    for label, queue in (("per_gate_launch", False), ("queued", True)):
+        s = DenseHip(n, queue=queue)
+        lib, h = s._lib, s.handle
+        best = 1e9
+        for _ in range(reps + 1):
+            s.clear()
+            s.synchronize()
+            t0 = time.perf_counter()
+            for k, a, t, m in rows:
+                lib.kbwh_gate_mask(h, k, a, t, m)
+            res = s.measure(list(range(n)), u)
+            dt = time.perf_counter() - t0
+            best = min(best, dt)
+        # the key packs the last listed qubit as bit 0: the target index reversed
+        assert res == int(format(target, f"0{n}b")[::-1], 2), (res, target)
+        out[label] = {"s": best, "gates_per_s": len(rows) / best, "outcome": int(res)}
+        s.close()
+    out["speedup_queued_vs_per_gate_launch"] = out["per_gate_launch"]["s"] / out["queued"]["s"]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = oracle.DenseV2(n)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        o.run(ops)
+    dt = (time.perf_counter() - t0) / reps
+    out["cpu_baseline"] = {"value": len(rows) / dt, "unit": "gates/s", "cores": threads, "kind": "port",
+                           "sample": f"the same {len(rows)} gates on the oracle, {reps} runs"}
+    return out
+
+
 def cpu_baseline(n, ops, ngates):
     import oracle
 
@@ -384,6 +430,8 @@ def main():
         result["adjoint"] = adjoint_leg()
     if "batch" in legs:
         result["batch"] = batch_leg()
+    if "live" in legs:
+        result["live12"] = live_leg()
     if "c2" in legs:
         result["block33"] = block_leg()
     if args.cpu_gates > 0 and "cpu" in legs:

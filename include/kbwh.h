@@ -58,6 +58,9 @@ extern "C" {
 
 /* kbwh_new flags */
 #define KBWH_FLAG_BLOCK 1u  /* Block<_, DenseV2> semantics (block.rs): logical/physical map, LRU swaps */
+#define KBWH_FLAG_QUEUE 2u  /* kbwh_gate queues gates; they are applied (same result bits) before any read of the
+                               state: one launch for up to 12 qubits, tiled passes above.  Not with KBWH_FLAG_BLOCK.
+                               kbwh_device_buffer users call kbwh_synchronize first. */
 
 /* kbwh_run modes */
 #define KBWH_RUN_PER_GATE 0u /* one HBM pass per gate: the plain path */

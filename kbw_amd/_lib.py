@@ -22,7 +22,7 @@ ERR_CAPACITY = 7
 X, Y, Z, H, P, RX, RY, RZ = range(8)
 GATE_NAMES = {"x": X, "y": Y, "z": Z, "h": H, "p": P, "rx": RX, "ry": RY, "rz": RZ}
 PAULI_X, PAULI_Y, PAULI_Z = 1, 2, 3
-FLAG_BLOCK = 1
+FLAG_BLOCK, FLAG_QUEUE = 1, 2
 RUN_PER_GATE, RUN_TILED, RUN_FUSED, RUN_AUTO = 0, 1, 2, 3
 
 # kbwh_op, 24 bytes

@@ -187,6 +187,7 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
     if (!s || !energy || (num_ops && !ops) || (num_params && (!params || !grad)) ||
         (num_strings && (!offsets || !coeffs)))
         return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    FLUSH(s);
     if (s->block) return set_error(KBWH_ERR_INVALID_ARGUMENT, "adjoint gradient needs a flat state");
     const uint32_t n = s->n;
     const uint64_t size = s->size;

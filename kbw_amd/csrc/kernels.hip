@@ -15,6 +15,7 @@
 
 #include <algorithm>
 
+#include "gate_math.h"
 #include "internal.h"
 
 #pragma clang fp contract(off)
@@ -53,43 +54,9 @@ __device__ __forceinline__ uint64_t expand1(uint64_t k, uint64_t mlo) {
     return ((k ^ lo) << 1) | lo;
 }
 
-// num-complex `a *= b`: re = re*br - im*bi; im = im*br + re*bi
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    double2 r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.y * b.x + a.x * b.y;
-    return r;
-}
-
-enum { KX = 0, KY = 1, KZ = 2, KH = 3, KP = 4, KRX = 5, KRY = 6, KRZ = 7 };
-
-template <int KIND>
-__device__ __forceinline__ void pair_math(double2 &k0, double2 &k1, double c, double s) {
-    const double2 t0 = k0, t1 = k1;
-    if constexpr (KIND == KX) {  // dense_v2.rs:168-170
-        k0 = t1;
-        k1 = t0;
-    } else if constexpr (KIND == KY) {  // dense_v2.rs:178-182: swap, *= -i, *= i
-        k0 = cmul(t1, make_double2(-0.0, -1.0));
-        k1 = cmul(t0, make_double2(0.0, 1.0));
-    } else if constexpr (KIND == KH) {  // dense_v2.rs:200-205
-        const double f = 0.70710678118654752440;
-        k0.x = (t0.x + t1.x) * f;
-        k0.y = (t0.y + t1.y) * f;
-        k1.x = (t0.x - t1.x) * f;
-        k1.y = (t0.y - t1.y) * f;
-    } else if constexpr (KIND == KRX) {  // dense_v2.rs:229-237
-        k0.x = c * t0.x + s * t1.y;
-        k0.y = c * t0.y - s * t1.x;
-        k1.x = s * t0.y + c * t1.x;
-        k1.y = -s * t0.x + c * t1.y;
-    } else if constexpr (KIND == KRY) {  // dense_v2.rs:250-258
-        k0.x = c * t0.x - s * t1.x;
-        k0.y = c * t0.y - s * t1.y;
-        k1.x = s * t0.x + c * t1.x;
-        k1.y = s * t0.y + c * t1.y;
-    }
-}
+using gm::cmul;
+using gm::pair_math;
+enum { KX = gm::KX, KY = gm::KY, KZ = gm::KZ, KH = gm::KH, KP = gm::KP, KRX = gm::KRX, KRY = gm::KRY, KRZ = gm::KRZ };
 
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_pair(double2 *__restrict__ a, uint64_t npairs,

@@ -72,6 +72,7 @@ int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qub
                        const uint32_t *offsets, const double *coeffs, uint32_t ns, double *out) {
     if (!s || !out || (ns && (!offsets || !coeffs)))
         return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    FLUSH(s);
     if (ns && offsets[ns] > offsets[0] && (!paulis || !qubits))
         return set_error(KBWH_ERR_INVALID_ARGUMENT, "null Pauli arrays");
     for (uint32_t k = 0; k < ns; ++k)
@@ -107,6 +108,7 @@ int32_t kbwh_dump(kbwh_state *s, const uint32_t *qubits, uint32_t nq, uint64_t c
                   uint64_t *count, uint64_t *keys, double *re, double *im) {
     if (!s || !count || (nq && !qubits)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
     if (keys && (!re || !im)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null output arrays");
+    FLUSH(s);
     QubitList ql;
     int32_t rc = physical_list(s, qubits, nq, &ql);
     if (rc) return rc;
@@ -151,6 +153,7 @@ int32_t kbwh_sample(kbwh_state *s, const uint32_t *qubits, uint32_t nq, uint64_t
                     const double *uniforms, uint64_t *outcomes) {
     if (!s || !qubits || (shots && (!uniforms || !outcomes)))
         return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    FLUSH(s);
     if (nq == 0)  // quantum_execution.rs:295 asserts a non-empty qubit list
         return set_error(KBWH_ERR_INVALID_ARGUMENT, "sample needs at least one qubit");
     QubitList ql;

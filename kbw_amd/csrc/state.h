@@ -40,6 +40,9 @@ struct kbwh_state {
     uint32_t fusion_width = 5;   // qubits per fused block
     uint32_t tile_targets = 8;   // non-diagonal targets above qubit 2 per tiled pass
     uint32_t auto_mode = KBWH_RUN_TILED;
+    // live gates queued with KBWH_FLAG_QUEUE, applied before the state is read
+    bool use_queue = false;
+    std::vector<kbwh_op> queue;
     uint64_t last_gates = 0;     // gates of the last kbwh_run
     // adjoint differentiation: two extra state vectors, allocated on first use
     double2 *adj_phi = nullptr;
@@ -62,6 +65,8 @@ int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const dou
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
                      const double *params);
 void free_adjoint(kbwh_state *s);
+int32_t queue_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint64_t ctrl_mask);
+int32_t flush_queue(kbwh_state *s);
 int32_t run_fused(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops, const double *params,
                   int k);
 
@@ -74,3 +79,10 @@ int32_t pauli_masks(const kbwh_state *s, const uint8_t *paulis, const uint32_t *
                     uint32_t lo, uint32_t hi, bool physical, PauliMasks *out);
 
 }  // namespace kbwh
+
+// Apply queued live gates before the state is read or replaced.
+#define FLUSH(s)                                        \
+    do {                                                \
+        const int32_t frc_ = kbwh::flush_queue(s);      \
+        if (frc_) return frc_;                          \
+    } while (0)

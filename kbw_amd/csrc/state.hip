@@ -179,6 +179,7 @@ int32_t kbwh_new(uint32_t num_qubits, uint32_t flags, uint32_t block_size, kbwh_
     if (!out) return fail(KBWH_ERR_INVALID_ARGUMENT, "out is null");
     *out = nullptr;
     const bool block = (flags & KBWH_FLAG_BLOCK) != 0;
+    if (block && (flags & KBWH_FLAG_QUEUE)) return fail(KBWH_ERR_INVALID_ARGUMENT, "the gate queue is for states without blocks");
     uint32_t nlocal = num_qubits;
     if (block) {
         if (num_qubits <= 1) return fail(KBWH_ERR_EXECUTION_FAILED, "Block needs at least 2 qubits");
@@ -196,6 +197,7 @@ int32_t kbwh_new(uint32_t num_qubits, uint32_t flags, uint32_t block_size, kbwh_
     s->n = num_qubits;
     s->block = block;
     s->block_size_arg = block_size;
+    s->use_queue = (flags & KBWH_FLAG_QUEUE) != 0;
     s->nlocal = nlocal;
     s->size = 1ull << num_qubits;
     hipError_t e = hipGetDevice(&s->device);
@@ -236,6 +238,7 @@ int32_t kbwh_free(kbwh_state *s) {
 
 int32_t kbwh_clear(kbwh_state *s) {
     if (!s) return fail(KBWH_ERR_INVALID_ARGUMENT, "null handle");
+    s->queue.clear();
     HIPCHK(launch_init(s->amp, s->size, s->stream));
     for (uint32_t i = 0; i < s->n; ++i) s->l2p[i] = s->p2l[i] = i;
     std::fill(s->last_access.begin(), s->last_access.end(), 0);
@@ -257,18 +260,19 @@ int32_t kbwh_gate(kbwh_state *s, int32_t kind, double angle, uint32_t target,
         if (control[i] >= s->n) return fail(KBWH_ERR_INVALID_ARGUMENT, "control out of range");
         mask |= 1ull << control[i];
     }
-    return apply_gate(s, kind, angle, target, mask);
+    return s->use_queue ? queue_gate(s, kind, angle, target, mask) : apply_gate(s, kind, angle, target, mask);
 }
 
 int32_t kbwh_gate_mask(kbwh_state *s, int32_t kind, double angle, uint32_t target,
                        uint64_t ctrl_mask) {
     if (!s) return fail(KBWH_ERR_INVALID_ARGUMENT, "null handle");
-    return apply_gate(s, kind, angle, target, ctrl_mask);
+    return s->use_queue ? queue_gate(s, kind, angle, target, ctrl_mask) : apply_gate(s, kind, angle, target, ctrl_mask);
 }
 
 int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
                  uint32_t mode) {
     if (!s || (num_ops && !ops)) return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    FLUSH(s);
     if (mode == KBWH_RUN_AUTO) mode = s->auto_mode;
     if (mode == KBWH_RUN_TILED) return run_tiled(s, ops, num_ops, params);
     if (mode == KBWH_RUN_FUSED) return run_fused(s, s->amp, ops, num_ops, params, (int)s->fusion_width);
@@ -285,11 +289,13 @@ int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const doub
 
 int32_t kbwh_measure_p1(kbwh_state *s, uint32_t target, double *p1) {
     if (!s || !p1) return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    FLUSH(s);
     return measure_p1(s, target, p1);
 }
 
 int32_t kbwh_measure_collapse(kbwh_state *s, uint32_t target, int32_t result, double p) {
     if (!s) return fail(KBWH_ERR_INVALID_ARGUMENT, "null handle");
+    FLUSH(s);
     return measure_collapse(s, target, result, p);
 }
 
@@ -298,6 +304,7 @@ int32_t kbwh_measure(kbwh_state *s, const uint32_t *qubits, uint32_t nq, const d
                      uint64_t *result) {
     if (!s || !result || (nq && (!qubits || !uniforms)))
         return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    FLUSH(s);
     if (nq > 64) return fail(KBWH_ERR_INVALID_ARGUMENT, "more than 64 qubits");
     uint64_t out = 0;
     for (uint32_t idx = 0; idx < nq; ++idx) {
@@ -325,6 +332,7 @@ int32_t kbwh_read_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, doub
     if (!s || (count && !out)) return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
     if (first > s->size || count > s->size - first)
         return fail(KBWH_ERR_INVALID_ARGUMENT, "range out of bounds");
+    FLUSH(s);
     HIPCHK(hipMemcpyAsync(out, s->amp + first, count * sizeof(double2), hipMemcpyDeviceToHost,
                           s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -335,6 +343,7 @@ int32_t kbwh_write_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, con
     if (!s || (count && !in)) return fail(KBWH_ERR_INVALID_ARGUMENT, "null argument");
     if (first > s->size || count > s->size - first)
         return fail(KBWH_ERR_INVALID_ARGUMENT, "range out of bounds");
+    FLUSH(s);
     HIPCHK(hipMemcpyAsync(s->amp + first, in, count * sizeof(double2), hipMemcpyHostToDevice,
                           s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -362,6 +371,7 @@ int32_t kbwh_run_stats(const kbwh_state *s, uint64_t *gates, uint64_t *passes) {
 
 int32_t kbwh_synchronize(kbwh_state *s) {
     if (!s) return fail(KBWH_ERR_INVALID_ARGUMENT, "null handle");
+    FLUSH(s);
     HIPCHK(hipStreamSynchronize(s->stream));
     return KBWH_OK;
 }

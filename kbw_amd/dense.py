@@ -75,10 +75,13 @@ def ops_array(ops):
 class DenseHip:
     """QuantumExecution on one MI355X: the dense c128 state lives in HBM."""
 
-    def __init__(self, num_qubits, block=False, block_size=0):
+    def __init__(self, num_qubits, block=False, block_size=0, queue=False):
+        """queue=True: live gates are queued and applied before the next read
+        of the state (KBWH_FLAG_QUEUE; one launch for up to 12 qubits)."""
         lib = _lib.load()
         h = C.c_void_p()
-        check(lib.kbwh_new(num_qubits, _lib.FLAG_BLOCK if block else 0, block_size, C.byref(h)))
+        flags = (_lib.FLAG_BLOCK if block else 0) | (_lib.FLAG_QUEUE if queue else 0)
+        check(lib.kbwh_new(num_qubits, flags, block_size, C.byref(h)))
         self._lib = lib
         self._h = h
         self.num_qubits = num_qubits
@@ -289,11 +292,11 @@ class QubitManager:
     reference uses rand's StdRng; the engine only consumes the uniforms).
     """
 
-    def __init__(self, num_qubits, block=False, block_size=0, seed=None):
+    def __init__(self, num_qubits, block=False, block_size=0, seed=None, queue=False):
         self.num_qubits = num_qubits
         self.block = block
         self.block_size = block_size
-        self.simulator = DenseHip(num_qubits, block=block, block_size=block_size)
+        self.simulator = DenseHip(num_qubits, block=block, block_size=block_size, queue=queue)
         s = kbw_seed() if seed is None else seed
         self.rng = np.random.default_rng(s)
 

@@ -109,21 +109,22 @@ int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const doub
 
 /* Run configuration (no counterpart in the reference).  fusion_width: qubits
  * per fused block for KBWH_RUN_FUSED, 4 or 5 (default 5).  tile_targets:
- * most non-diagonal targets outside qubits 0..2 per tiled pass, 1..8
- * (default 8, the tile's free positions). */
+ * most non-diagonal targets outside qubits 0..2 per tiled pass, 1..64
+ * (default 64: as many as the tile's free positions, 8 for 11-qubit tiles). */
 int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_targets);
 
 /* Autotuner (no counterpart in the reference): times each candidate
  * configuration of `ops` (at most the first 256 gates) on a scratch buffer
  * the size of the state, leaving the state untouched, and keeps the fastest
- * for KBWH_RUN_AUTO.  Candidates: per-gate; tiled with 8, 6 and 4 targets
- * per pass; with allow_inexact, fused blocks of 4 and 5 qubits.  ms[i] is the
+ * for KBWH_RUN_AUTO.  Candidates: per-gate; tiled with all free positions,
+ * 6 and 4 targets per pass; with allow_inexact, fused blocks of 4 and 5
+ * qubits.  ms[i] is the
  * time of candidate i in that order (0 when not run).  Needs device memory
  * for a second state (KBWH_ERR_OUT_OF_MEMORY otherwise). */
 typedef struct kbwh_tune_result {
     uint32_t mode;          /* KBWH_RUN_PER_GATE, KBWH_RUN_TILED or KBWH_RUN_FUSED */
     uint32_t fusion_width;  /* for KBWH_RUN_FUSED */
-    uint32_t tile_targets;  /* for KBWH_RUN_TILED */
+    uint32_t tile_targets;  /* for KBWH_RUN_TILED (64: no cap) */
     uint32_t num_candidates;
     double ms[8];
 } kbwh_tune_result;

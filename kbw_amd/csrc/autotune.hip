@@ -24,8 +24,8 @@ struct Candidate {
 };
 
 const Candidate kCandidates[] = {
-    {KBWH_RUN_PER_GATE, 5, 8, true}, {KBWH_RUN_TILED, 5, 8, true}, {KBWH_RUN_TILED, 5, 6, true},
-    {KBWH_RUN_TILED, 5, 4, true},    {KBWH_RUN_FUSED, 4, 8, false}, {KBWH_RUN_FUSED, 5, 8, false},
+    {KBWH_RUN_PER_GATE, 5, 64, true}, {KBWH_RUN_TILED, 5, 64, true}, {KBWH_RUN_TILED, 5, 6, true},
+    {KBWH_RUN_TILED, 5, 4, true},     {KBWH_RUN_FUSED, 4, 64, false}, {KBWH_RUN_FUSED, 5, 64, false},
 };
 
 int32_t run_on(kbwh_state *s, double2 *buf, const Candidate &c, const kbwh_op *ops, uint64_t n,
@@ -61,7 +61,7 @@ extern "C" {
 int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_targets) {
     if (!s) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null handle");
     if (fusion_width != 4 && fusion_width != 5) return set_error(KBWH_ERR_INVALID_ARGUMENT, "fusion width must be 4 or 5");
-    if (tile_targets < 1 || tile_targets > 8) return set_error(KBWH_ERR_INVALID_ARGUMENT, "tile targets must be 1..8");
+    if (tile_targets < 1 || tile_targets > 64) return set_error(KBWH_ERR_INVALID_ARGUMENT, "tile targets must be 1..64");
     s->fusion_width = fusion_width;
     s->tile_targets = tile_targets;
     return KBWH_OK;

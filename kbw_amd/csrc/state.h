@@ -38,7 +38,7 @@ struct kbwh_state {
     uint64_t last_passes = 0;    // HBM passes of the last kbwh_run
     // run configuration (kbwh_set_run_config / kbwh_autotune)
     uint32_t fusion_width = 5;   // qubits per fused block
-    uint32_t tile_targets = 8;   // non-diagonal targets above qubit 2 per tiled pass
+    uint32_t tile_targets = 64;  // most non-diagonal targets above qubit 2 per tiled pass (64: the tile's free positions)
     uint32_t auto_mode = KBWH_RUN_TILED;
     // live gates queued with KBWH_FLAG_QUEUE, applied before the state is read
     bool use_queue = false;

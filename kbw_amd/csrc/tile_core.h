@@ -16,6 +16,9 @@
 
 namespace kbwh {
 namespace tile {
+// Internal linkage: a translation unit may be built with its own tile width
+// (KBWH_TILE_BITS), and no tile type crosses translation units.
+namespace {
 
 // Tile shape, measured on MI355X for the 30-qubit brickwork circuit: 2^11
 // amplitudes per tile (8 per thread), one tile per workgroup and 4 waves per
@@ -467,5 +470,6 @@ inline void add_op(Pass &p, const PhysOp &o) {
     p.ops.push_back(o);
 }
 
+}  // namespace
 }  // namespace tile
 }  // namespace kbwh

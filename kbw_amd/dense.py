@@ -148,8 +148,9 @@ class DenseHip:
         p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
         check(self._lib.kbwh_run(self._h, ptr(a), len(a), ptr(p), mode))
 
-    def set_run_config(self, fusion_width=5, tile_targets=8):
-        """Qubits per fused block (4 or 5) and non-diagonal targets per tiled pass (1..8)."""
+    def set_run_config(self, fusion_width=5, tile_targets=64):
+        """Qubits per fused block (4 or 5) and most non-diagonal targets per
+        tiled pass (1..64; 64 = as many as the tile has free positions)."""
         check(self._lib.kbwh_set_run_config(self._h, fusion_width, tile_targets))
 
     def autotune(self, ops, params=None, allow_inexact=False):

@@ -72,7 +72,7 @@ def test_fused_rejects_bad_config_and_block_states():
     with pytest.raises(Exception):
         g.set_run_config(fusion_width=3)
     with pytest.raises(Exception):
-        g.set_run_config(tile_targets=9)
+        g.set_run_config(tile_targets=65)
     b = DenseHip(14, block=True, block_size=12)
     ops = circuits.random_circuit(np.random.default_rng(1), 14, 10)
     with pytest.raises(Exception):
@@ -111,7 +111,7 @@ def test_tile_targets_cap_keeps_results_exact():
     o = oracle.DenseV2(n)
     o.run(ops)
     passes = []
-    for cap in (8, 4, 1):
+    for cap in (64, 4, 1):
         g = DenseHip(n)
         g.set_run_config(tile_targets=cap)
         g.run(ops, mode=RUN_TILED)

@@ -23,7 +23,8 @@ parameter shift on the plain path, plus 256 parameter sets at 16 qubits),
 "batch" (4096 ansatz circuits of 10-20 qubits in one persistent launch,
 circuits/s), "live12" (the 12-qubit Grover circuit given gate by gate:
 one launch per gate against queued gates in one launch) and, on request,
-"block33" (the 33-qubit blocked QFT circuit).
+"block33" (the 33-qubit blocked QFT circuit) and "fused" (the brickwork
+circuit in FP64 MFMA blocks of 4 and 5 qubits).
 """
 import argparse
 import json
@@ -52,7 +53,7 @@ def parse():
     ap.add_argument("--mode", choices=["per_gate", "tiled", "best"], default="best")
     ap.add_argument("--cpu-gates", type=int, default=150, help="oracle sample size (gates); 0 = skip")
     ap.add_argument("--legs", default="tiled,per_gate,exp,adjoint,batch,live,cpu",
-                    help="comma list of legs: tiled, per_gate, exp, adjoint, batch, live, cpu, c2")
+                    help="comma list of legs: tiled, per_gate, exp, adjoint, batch, live, cpu, c2, fused")
     ap.add_argument("--pmc", action="store_true",
                     help="counter run: a 1 GiB torch copy then 4 pair passes (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -241,6 +242,48 @@ def batch_leg(num=4096, reps=3, single_sample=64, cpu_sample=96):
                                          f"(OpenMP {threads} threads), {dt:.2f} s, scaled to the batch "
                                          f"by total amplitudes"}
     return res
+
+
+def fused_blocks(ops, k):
+    """Blocks of the fused mode for a gate list (the engine's greedy rule:
+    consecutive gates while their qubits fit in k; wider gates alone)."""
+    blocks, cur = 0, 0
+    for o in ops:
+        supp = int(o["ctrl_mask"]) | (1 << int(o["target"]))
+        if bin(supp).count("1") > k:
+            blocks += 1 if cur else 0
+            cur = 0
+            continue
+        if bin(cur | supp).count("1") > k:
+            blocks += 1
+            cur = 0
+        cur |= supp
+    return blocks + (1 if cur else 0)
+
+
+def fused_leg(n=30, layers=20):
+    """The brickwork circuit in fused k-qubit blocks on FP64 MFMA (k = 4, 5):
+    time per run and the achieved FP64 rate, counting 8 * 2^k flops per
+    amplitude per block (the dense 2^(k+1)-square real product)."""
+    from kbw_amd import DenseHip, RUN_FUSED, circuits
+
+    ops, _ = circuits.brickwork(n, layers, seed=1)
+    s = DenseHip(n)
+    out = {"workload": f"brickwork-{n}q-c128-{layers}layers-seed1", "gates": len(ops)}
+    for k in (4, 5):
+        s.set_run_config(fusion_width=k)
+        s.run(ops, mode=RUN_FUSED)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.run(ops, mode=RUN_FUSED)
+        s.synchronize()
+        dt = time.perf_counter() - t0
+        nb = fused_blocks(ops, k)
+        flops = nb * 8.0 * (1 << k) * (1 << n)
+        out[f"k{k}"] = {"s": dt, "gates_per_s": len(ops) / dt, "passes": s.run_stats()[1], "blocks": nb,
+                        "tflops": flops / dt / 1e12, "mfma_peak_tflops": 78.6}
+    s.close()
+    return out
 
 
 def live_leg(reps=5):
@@ -432,6 +475,8 @@ def main():
         result["batch"] = batch_leg()
     if "live" in legs:
         result["live12"] = live_leg()
+    if "fused" in legs:
+        result["fused"] = fused_leg()
     if "c2" in legs:
         result["block33"] = block_leg()
     if args.cpu_gates > 0 and "cpu" in legs:

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
             const uint32_t flags = P.flags;
             uint64_t tmask = 0;
 #pragma unroll
-            for (int b = 0; b < 8; ++b)
+            for (int b = 0; b < TB; ++b)
                 if ((tid >> b) & 1u) tmask |= P.ts.qbit[b];
             for (uint32_t t = 0; t < ntiles; ++t) {
                 const uint64_t base = tile_base(t, P.ts);
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
                     if (flags & kPassLds) {
                         __syncthreads();  // earlier exchange reads are done
 #pragma unroll
-                        for (int r = 0; r < TR; ++r) lds[tid | (r << 8)] = v[r];
+                        for (int r = 0; r < TR; ++r) lds[tid | (r << TB)] = v[r];
                         __syncthreads();
                     }
                     double acc = 0.0;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
                             if (S.xt == 0) {
                                 term = v[r].x * v[r].x + v[r].y * v[r].y;
                             } else {
-                                const dv2 o = lds[(tid | (r << 8)) ^ S.xt], m = v[r];
+                                const dv2 o = lds[(tid | (r << TB)) ^ S.xt], m = v[r];
                                 const double wr = o.x * m.x + o.y * m.y;
                                 const double wi = o.x * m.y - o.y * m.x;
                                 const uint32_t q = S.ny & 3u;

@@ -26,6 +26,11 @@
 
 #include "internal.h"
 #include "state.h"
+// the block kernel is written for 11-qubit tiles of 256 threads (4 waves)
+#undef KBWH_TILE_BITS
+#undef KBWH_TILE_TBITS
+#define KBWH_TILE_BITS 11
+#define KBWH_TILE_TBITS 8
 #include "tile_core.h"
 
 #pragma clang fp contract(off)

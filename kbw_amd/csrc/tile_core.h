@@ -29,9 +29,14 @@ namespace {
 #ifndef KBWH_TILE_WAVES
 #define KBWH_TILE_WAVES 4
 #endif
-constexpr int TL = KBWH_TILE_BITS;  // tile qubits
-constexpr int TT = 256;             // threads per tile
-constexpr int TR = 1 << (TL - 8);   // amplitudes per thread (register bits = TL - 8)
+#ifndef KBWH_TILE_TBITS
+#define KBWH_TILE_TBITS 8
+#endif
+constexpr int TL = KBWH_TILE_BITS;   // tile qubits
+constexpr int TB = KBWH_TILE_TBITS;  // thread bits: 6 lane bits, then TB - 6 wave bits
+constexpr int TT = 1 << TB;          // threads per tile
+constexpr int TR = 1 << (TL - TB);   // amplitudes per thread (register bits = TL - TB)
+static_assert(TB >= 6 && TL - TB >= 1 && TL - TB <= 4, "tile geometry");
 constexpr uint32_t kOutside = 255;
 constexpr int kChunk = 4;        // register slots per LDS exchange round (16 KiB of LDS)
 
@@ -125,11 +130,14 @@ __device__ __forceinline__ dv2 pair_out(dv2 m, dv2 o, const PairCoef &k) {
 enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2 };
 
 // Control test for the amplitude in register slot r of thread tid, whose tile
-// index is tid + 256 r: the thread bits are tested once per gate (tok) and the
+// index is tid + TT r: the thread bits are tested once per gate (tok) and the
 // slot bits against the uniform part of cm, so no per-slot index is kept.
-__device__ __forceinline__ bool thread_ctrl_ok(uint32_t tid, uint32_t cm) { return (tid & (cm & 0xFFu)) == (cm & 0xFFu); }
+__device__ __forceinline__ bool thread_ctrl_ok(uint32_t tid, uint32_t cm) {
+    const uint32_t lo = cm & (uint32_t)(TT - 1);
+    return (tid & lo) == lo;
+}
 __device__ __forceinline__ bool slot_ctrl_ok(bool tok, int r, uint32_t cm) {
-    const uint32_t hi = cm >> 8;
+    const uint32_t hi = cm >> TB;
     return tok && (((uint32_t)r & hi) == hi);
 }
 
@@ -199,11 +207,11 @@ __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid
     for (int q = 0; q < TR; q += kChunk) {
         __syncthreads();  // earlier exchange reads are done
 #pragma unroll
-        for (int r = 0; r < kChunk; ++r) lds[tid + (r << 8)] = v[q + r];
+        for (int r = 0; r < kChunk; ++r) lds[tid + (r << TB)] = v[q + r];
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < kChunk; ++r) {
-            const dv2 o = lds[pt + (r << 8)];
+            const dv2 o = lds[pt + (r << TB)];
             const dv2 out = pair_val<PK>(v[q + r], o, one, G);
             if constexpr (CTRL) {
                 v[q + r] = slot_ctrl_ok(tok, q + r, cm) ? out : v[q + r];
@@ -217,24 +225,32 @@ __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid
 template <int PK, bool CTRL>
 __device__ __forceinline__ void apply_pair_t(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
     const uint32_t tp = G.tpos;
-    if (tp >= 8) {
-        // Register bit rb = tp - 8, dispatched on its bits.  A `switch (tp)`
+    if (tp >= (uint32_t)TB) {
+        // Register bit rb = tp - TB, dispatched on its bits.  A `switch (tp)`
         // here was lowered by the gfx950 backend into a compare tree whose
         // default case had two predecessors, and the structurized code never
         // ran that case (the last register bit's gates were dropped).
-        const uint32_t rb = tp - 8;
-        if (rb & 2u) {
-#if KBWH_TILE_BITS > 11
-            if (rb & 1u)
-                reg_pair<3, PK, CTRL>(v, tid, G);
-            else
-#endif
-                reg_pair<2, PK, CTRL>(v, tid, G);
-        } else if (rb & 1u) {
-            reg_pair<1, PK, CTRL>(v, tid, G);
-        } else {
-            reg_pair<0, PK, CTRL>(v, tid, G);
+        const uint32_t rb = tp - TB;
+        if constexpr (TL - TB > 2) {
+            if (rb & 2u) {
+                if constexpr (TL - TB > 3) {
+                    if (rb & 1u)
+                        reg_pair<3, PK, CTRL>(v, tid, G);
+                    else
+                        reg_pair<2, PK, CTRL>(v, tid, G);
+                } else {
+                    reg_pair<2, PK, CTRL>(v, tid, G);
+                }
+                return;
+            }
         }
+        if constexpr (TL - TB > 1) {
+            if (rb & 1u) {
+                reg_pair<1, PK, CTRL>(v, tid, G);
+                return;
+            }
+        }
+        reg_pair<0, PK, CTRL>(v, tid, G);
         return;
     }
     thread_pair<PK, CTRL>(v, lds, tid, tp, G);
@@ -305,8 +321,8 @@ __device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32
     const uint32_t cm = G.cm, tp = G.tpos;
     const bool rz = G.kind == KRZ;
     // one range test and a select (no multi-way branch on tp; see apply_pair_t)
-    if (tp - 8u < 8u) {
-        diag_reg<NEG, CTRL>(v, tid, cm, tp - 8u, rz, ph0, ph1);
+    if (tp - (uint32_t)TB < 8u) {
+        diag_reg<NEG, CTRL>(v, tid, cm, tp - (uint32_t)TB, rz, ph0, ph1);
         return;
     }
     // the target bit is fixed per tile (outside Q) or per thread (lane/wave bit)
@@ -341,8 +357,8 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t k, const TileSpec &ts) {
 __device__ __forceinline__ uint64_t slot_offset(uint64_t dt, int r, const TileSpec &ts) {
     uint64_t o = dt;
 #pragma unroll
-    for (int b = 0; b < TL - 8; ++b)
-        if ((r >> b) & 1) o |= ts.qbit[8 + b];
+    for (int b = 0; b < TL - TB; ++b)
+        if ((r >> b) & 1) o |= ts.qbit[TB + b];
     return o;
 }
 
@@ -363,7 +379,7 @@ struct Pass {
 };
 
 // Tile positions for a pass: qubits 0..2 on lane bits 0..2; the most used
-// targets on register bits, then lane bits 3..5, then wave bits 6..7; free
+// targets on register bits, then lane bits 3..5, then wave bits; free
 // positions take the lowest qubits not used yet.
 inline void assign_positions(uint32_t n, const Pass &p, int pos_of[64], uint32_t q_at[TL]) {
     for (int i = 0; i < 64; ++i) pos_of[i] = -1;
@@ -387,8 +403,8 @@ inline void assign_positions(uint32_t n, const Pass &p, int pos_of[64], uint32_t
     int order[TL - 3], fill_order[TL - 3];
     {
         int o = 0;
-        for (int b = 8; b < TL; ++b) order[o++] = b;  // register bits first
-        for (int b = 3; b < 8; ++b) order[o++] = b;   // then lane bits 3..5, wave bits 6..7
+        for (int b = TB; b < TL; ++b) order[o++] = b;  // register bits first
+        for (int b = 3; b < TB; ++b) order[o++] = b;   // then lane bits 3..5, then wave bits
         for (int b = 0; b < TL - 3; ++b) fill_order[b] = 3 + b;
     }
     size_t k = 0;

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     if (tile >= ntiles) return;
     uint64_t tmask = 0;
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
+    for (int b = 0; b < TB; ++b)
         if ((tid >> b) & 1u) tmask |= ts.qbit[b];
     const uint64_t base = tile_base(tile, ts);
     const uint64_t dt = base | tmask;

@@ -192,16 +192,19 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
                         for (int r = 0; r < TR; ++r) {
                             const bool neg = __popcll(slot_offset(dt, r, P.ts) & S.z) & 1;
                             double term;
+                            bool flip = neg;
                             if (S.xt == 0) {
                                 term = v[r].x * v[r].x + v[r].y * v[r].y;
                             } else {
                                 const dv2 o = lds[(tid | (r << TB)) ^ S.xt], m = v[r];
                                 const double wr = o.x * m.x + o.y * m.y;
                                 const double wi = o.x * m.y - o.y * m.x;
+                                // Re(i^q (wr + i wi)): wr, -wi, -wr, wi for q = 0..3
                                 const uint32_t q = S.ny & 3u;
-                                term = (q == 0) ? wr : (q == 1) ? -wi : (q == 2) ? -wr : wi;
+                                term = (q & 1u) ? wi : wr;
+                                flip = flip != (((q + 1u) & 2u) != 0);
                             }
-                            sk += neg ? -term : term;
+                            sk += flip ? -term : term;
                         }
                         acc += S.coeff * sk;
                     }

@@ -5,87 +5,67 @@
 // queue flag, kbwh_gate validates the gate and appends it to the handle's
 // queue; the queue is flushed before anything reads or replaces the state.
 // A state of up to 12 qubits (64 KiB) is flushed in one launch: a single
-// 1024-thread workgroup holds the whole state in LDS and applies every queued
-// gate in order, a barrier between gates, with the per-gate kernels' exact
-// arithmetic, so results are the same bits as gate by gate.  Larger states
-// are flushed through the tiled planner (also exact); Block states apply the
-// queue gate by gate (their map and LRU follow each gate).
+// 1024-thread workgroup holds the whole state as one 12-qubit register tile
+// (4 amplitudes per thread: 6 lane bits, 4 wave bits, 2 register bits) and
+// applies every queued gate in order with the tiled pass's exact per-pair
+// arithmetic, so results are the same bits as gate by gate.  The planner puts
+// the most used targets on the register bits; only gates on wave bits touch
+// LDS.  A state narrower than 12 qubits is padded with qubits that no gate
+// touches (their amplitudes stay 0 and are neither loaded nor stored).
+// Larger states are flushed through the tiled planner (also exact).
 #include <string.h>
 
 #include <vector>
 
-#include "gate_math.h"
 #include "internal.h"
 #include "state.h"
+// the whole state of up to 12 qubits is one tile of 1024 threads
+#undef KBWH_TILE_BITS
+#undef KBWH_TILE_TBITS
+#undef KBWH_TILE_WAVES
+#define KBWH_TILE_BITS 12
+#define KBWH_TILE_TBITS 10
+#define KBWH_TILE_WAVES 4
+#include "tile_core.h"
 
 #pragma clang fp contract(off)
 
 namespace kbwh {
 namespace {
 
-constexpr uint32_t kLiveQubits = 12;
-constexpr int kLiveThreads = 1024;
+using namespace tile;
 
-struct LGate {
-    uint32_t kind, target;
-    uint32_t ctrl, pad;  // controls without the target (n <= 12)
-    double c, s;
-    double2 ph0, ph1;
-};
-static_assert(sizeof(LGate) == 64, "LGate layout");
+constexpr uint32_t kLiveQubits = TL;
 
-// The gate's kind is dispatched on its bits (nested two-way branches, one
-// predecessor per case) rather than with a switch.
-__device__ __forceinline__ void live_pair(double2 &t0, double2 &t1, const LGate &G) {
-    const uint32_t k = G.kind;
-    if (k & 4u) {
-        if (k & 2u) {
-            if (k & 1u)
-                gm::diag_math<gm::KRZ>(t0, t1, G.ph0, G.ph1);
-            else
-                gm::pair_math<gm::KRY>(t0, t1, G.c, G.s);
-        } else {
-            if (k & 1u)
-                gm::pair_math<gm::KRX>(t0, t1, G.c, G.s);
-            else
-                gm::diag_math<gm::KP>(t0, t1, G.ph0, G.ph1);
-        }
-    } else {
-        if (k & 2u) {
-            if (k & 1u)
-                gm::pair_math<gm::KH>(t0, t1, G.c, G.s);
-            else
-                gm::diag_math<gm::KZ>(t0, t1, G.ph0, G.ph1);
-        } else {
-            if (k & 1u)
-                gm::pair_math<gm::KY>(t0, t1, G.c, G.s);
-            else
-                gm::pair_math<gm::KX>(t0, t1, G.c, G.s);
-        }
+__global__ __launch_bounds__(TT) void k_whole(dv2 *__restrict__ a, uint32_t n, TileSpec ts,
+                                              const TGate *__restrict__ gates, uint32_t ng) {
+    __shared__ dv2 lds[TT * kChunk];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t size = 1ull << n;
+    uint64_t dt = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b)
+        if ((tid >> b) & 1u) dt |= ts.qbit[b];
+    dv2 v[TR];
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const uint64_t i = slot_offset(dt, r, ts);
+        v[r] = i < size ? a[i] : mk2(0.0, 0.0);
     }
-}
-
-__global__ __launch_bounds__(kLiveThreads) void k_live(double2 *__restrict__ a, uint32_t n,
-                                                        const LGate *__restrict__ gates, uint32_t ng) {
-    __shared__ double2 st[1u << kLiveQubits];
-    const uint32_t size = 1u << n, tid = threadIdx.x;
-    for (uint32_t i = tid; i < size; i += kLiveThreads) st[i] = a[i];
-    __syncthreads();
-    const uint32_t npairs = size >> 1;
-    for (uint32_t g = 0; g < ng; ++g) {
-        const LGate G = gates[g];
-        const uint32_t tb = 1u << G.target, lo = tb - 1u;
-        for (uint32_t p = tid; p < npairs; p += kLiveThreads) {
-            const uint32_t i = ((p & ~lo) << 1) | (p & lo);
-            if ((i & G.ctrl) != G.ctrl) continue;
-            double2 t0 = st[i], t1 = st[i | tb];
-            live_pair(t0, t1, G);
-            st[i] = t0;
-            st[i | tb] = t1;
-        }
-        __syncthreads();
+    // ng >= 1: the host flushes only a non-empty queue
+    uint32_t g = 0;
+    do {
+        const TGate G = gates[g];
+        if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
+            apply_diag(v, 0, tid, G);
+        else
+            apply_pair(v, lds, tid, G);
+    } while (++g < ng);
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const uint64_t i = slot_offset(dt, r, ts);
+        if (i < size) a[i] = v[r];
     }
-    for (uint32_t i = tid; i < size; i += kLiveThreads) a[i] = st[i];
 }
 
 }  // namespace
@@ -117,21 +97,20 @@ int32_t flush_queue(kbwh_state *s) {
         return KBWH_OK;
     }
     if (s->n > kLiveQubits) return run_tiled(s, q.data(), q.size(), nullptr);
-    std::vector<LGate> lg(q.size());
-    for (size_t i = 0; i < q.size(); ++i) {
-        const kbwh_op &op = q[i];
-        const Coef cf = make_coef(op.kind, op.angle);
-        LGate &g = lg[i];
-        memset(&g, 0, sizeof g);
-        g.kind = op.kind;
-        g.target = op.target;
-        g.ctrl = (uint32_t)(op.ctrl_mask & ~(1ull << op.target));
-        g.c = cf.c;
-        g.s = cf.s;
-        g.ph0 = cf.ph0;
-        g.ph1 = cf.ph1;
+    // one pass over a 12-qubit tile holding qubits 0..11 (n..11 are padding)
+    Pass all;
+    for (const kbwh_op &op : q) {
+        const PhysOp po{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), op.angle};
+        add_op(all, po);
     }
-    const size_t bytes = lg.size() * sizeof(LGate);
+    int pos_of[64];
+    uint32_t q_at[TL];
+    assign_positions(kLiveQubits, all, pos_of, q_at);
+    const TileSpec ts = make_tile_spec(q_at);
+    std::vector<TGate> tg;
+    tg.reserve(all.ops.size());
+    for (const PhysOp &o : all.ops) tg.push_back(make_tgate(o, pos_of, kLiveQubits));
+    const size_t bytes = tg.size() * sizeof(TGate);
     if (s->tile_gates_cap < bytes) {
         hipError_t e = hipStreamSynchronize(s->stream);
         if (e == hipSuccess && s->tile_gates) e = hipFree(s->tile_gates);
@@ -142,9 +121,10 @@ int32_t flush_queue(kbwh_state *s) {
         s->tile_gates_cap = bytes;
     }
     // the host copy is local: wait for the upload before it goes away
-    hipError_t e = hipMemcpyAsync(s->tile_gates, lg.data(), bytes, hipMemcpyHostToDevice, s->stream);
+    hipError_t e = hipMemcpyAsync(s->tile_gates, tg.data(), bytes, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) {
-        k_live<<<1, kLiveThreads, 0, s->stream>>>(s->amp, s->n, (const LGate *)s->tile_gates, (uint32_t)lg.size());
+        k_whole<<<1, TT, 0, s->stream>>>((dv2 *)s->amp, s->n, ts, (const TGate *)s->tile_gates,
+                                         (uint32_t)tg.size());
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);

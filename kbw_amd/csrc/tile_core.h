@@ -296,8 +296,9 @@ __device__ __forceinline__ void diag_all(dv2 (&v)[TR], uint32_t tid, uint32_t cm
 }
 
 // target on register bit rb: slot r is in the |1> half when bit rb of r is
-// set, a uniform test per slot (no switch over rb, whose near-identical cases
-// the compiler would merge into code that addresses v[] through pointers)
+// set.  Branch-free (every slot computed, kept by a uniform select): a branch
+// per slot on a runtime rb was turned back into a switch over the positions,
+// with pointer phis into v[] that demoted the register array to scratch.
 template <bool NEG, bool CTRL>
 __device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm, uint32_t rb, bool rz,
                                          dv2 ph0, dv2 ph1) {
@@ -305,13 +306,10 @@ __device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
         const bool one = ((uint32_t)r >> rb) & 1u;
-        if (!one && !rz) continue;
         const dv2 out = diag_val<NEG>(v[r], one ? ph1 : ph0);
-        if constexpr (CTRL) {
-            v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
-        } else {
-            v[r] = out;
-        }
+        bool apply = one || rz;
+        if constexpr (CTRL) apply = apply && slot_ctrl_ok(tok, r, cm);
+        v[r] = apply ? out : v[r];
     }
 }
 

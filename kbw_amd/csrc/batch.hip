@@ -164,13 +164,22 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
 #pragma unroll
                     for (int r = 0; r < TR; ++r) v[r] = a[slot_offset(dt, r, P.ts)];
                 }
-                for (uint32_t i = 0; i < P.num_gates; ++i) {
-                    const TGate G = gates[P.first_gate + i];
-                    if ((base & G.ocm) != G.ocm) continue;
-                    if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
-                        apply_diag(v, base, tid, G);
-                    else
-                        apply_pair(v, lds, tid, G);
+                // the next gate's record is loaded while the current gate is applied
+                const uint32_t ng = P.num_gates;
+                if (ng) {
+                    const TGate *pg = gates + P.first_gate;
+                    TGate G = pg[0];
+                    uint32_t i = 0;
+                    do {
+                        const TGate Gn = pg[i + 1 < ng ? i + 1 : i];
+                        if ((base & G.ocm) == G.ocm) {
+                            if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
+                                apply_diag(v, base, tid, G);
+                            else
+                                apply_pair(v, lds, tid, G);
+                        }
+                        G = Gn;
+                    } while (++i < ng);
                 }
                 if (flags & kPassStore) {
 #pragma unroll

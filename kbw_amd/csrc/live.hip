@@ -52,14 +52,17 @@ __global__ __launch_bounds__(TT) void k_whole(dv2 *__restrict__ a, uint32_t n, T
         const uint64_t i = slot_offset(dt, r, ts);
         v[r] = i < size ? a[i] : mk2(0.0, 0.0);
     }
-    // ng >= 1: the host flushes only a non-empty queue
+    // ng >= 1: the host flushes only a non-empty queue; the next gate's
+    // record is loaded while the current gate is applied
     uint32_t g = 0;
+    TGate G = gates[0];
     do {
-        const TGate G = gates[g];
+        const TGate Gn = gates[g + 1 < ng ? g + 1 : g];
         if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
             apply_diag(v, 0, tid, G);
         else
             apply_pair(v, lds, tid, G);
+        G = Gn;
     } while (++g < ng);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {

@@ -38,7 +38,10 @@ constexpr int TT = 1 << TB;          // threads per tile
 constexpr int TR = 1 << (TL - TB);   // amplitudes per thread (register bits = TL - TB)
 static_assert(TB >= 6 && TL - TB >= 1 && TL - TB <= 4, "tile geometry");
 constexpr uint32_t kOutside = 255;
-constexpr int kChunk = 4;        // register slots per LDS exchange round (16 KiB of LDS)
+#ifndef KBWH_TILE_CHUNK
+#define KBWH_TILE_CHUNK 4
+#endif
+constexpr int kChunk = KBWH_TILE_CHUNK < TR ? KBWH_TILE_CHUNK : TR;  // register slots per LDS exchange round
 
 struct TGate {
     uint32_t kind;

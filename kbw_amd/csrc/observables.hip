@@ -78,6 +78,42 @@ int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qub
     for (uint32_t k = 0; k < ns; ++k)
         if (offsets[k + 1] < offsets[k])
             return set_error(KBWH_ERR_INVALID_ARGUMENT, "offsets not ascending");
+    if (!s->block && s->n >= 11 && ns > 0) {
+        // grouped read-only sweeps, several strings per read of the state; a
+        // string with more than 8 X/Y qubits above qubit 2 is summed alone
+        std::vector<PauliMasks> grouped, alone;
+        std::vector<double> gc, ac;
+        for (uint32_t k = 0; k < ns; ++k) {
+            PauliMasks m;
+            int32_t rc = pauli_masks(s, paulis, qubits, offsets[k], offsets[k + 1], true, &m);
+            if (rc) return rc;
+            if (__builtin_popcountll(m.x | 7ull) <= 11) {
+                grouped.push_back(m);
+                gc.push_back(coeffs[k]);
+            } else {
+                alone.push_back(m);
+                ac.push_back(coeffs[k]);
+            }
+        }
+        double total = 0.0;
+        if (!grouped.empty()) {
+            int32_t rc = exp_value_tiled(s, grouped, gc.data(), &total);
+            if (rc) return rc;
+        }
+        for (size_t k = 0; k < alone.size(); ++k) {
+            uint32_t nb = 0;
+            hipError_t e = launch_pauli_partial(s->amp, s->n, alone[k].x, alone[k].z, alone[k].ny, s->partial, &nb,
+                                                s->stream);
+            if (e == hipSuccess) e = launch_sum_final(s->partial, nb, s->result, s->stream);
+            if (e != hipSuccess) return hip_status(e, "exp_value");
+            double v = 0.0;
+            int32_t rc = reduce_to_host(s, &v, 1);
+            if (rc) return rc;
+            total += v * ac[k];
+        }
+        *out = total;
+        return KBWH_OK;
+    }
     std::vector<double> values(ns, 0.0);
     uint32_t pending = 0, first = 0;
     for (uint32_t k = 0; k < ns; ++k) {

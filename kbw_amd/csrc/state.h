@@ -75,6 +75,7 @@ struct PauliMasks {
     uint64_t x, z;
     uint32_t ny;
 };
+int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out);
 int32_t pauli_masks(const kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
                     uint32_t lo, uint32_t hi, bool physical, PauliMasks *out);
 

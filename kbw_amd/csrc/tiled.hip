@@ -74,7 +74,161 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
 }
 
 
+// One Pauli string of a grouped exp_value sweep: P|j> = i^ny (-1)^popcount(j & z) |j ^ x>,
+// with x as a mask of tile positions (xt); coeff is the string's weight.
+struct SString {
+    uint64_t z;
+    uint32_t ny, xt;
+    double coeff;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *ws) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < TT / 64; ++w) t += ws[w];
+    __syncthreads();
+    return t;
+}
+
+// sum_k coeff_k <P_k> over the strings of one group, whose X/Y qubits are all
+// tile qubits of ts: each workgroup walks tiles t = blockIdx.x + j gridDim.x,
+// reads each tile once (partners through LDS) and writes one partial sum.
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_pauli_tile(
+    const dv2 *__restrict__ a, TileSpec ts, const SString *__restrict__ ss, uint32_t ns, uint32_t has_x,
+    uint32_t ntiles, double *__restrict__ partial) {
+    __shared__ dv2 lds[TT * TR];
+    __shared__ double ws[TT / 64];
+    const uint32_t tid = threadIdx.x;
+    uint64_t tmask = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b)
+        if ((tid >> b) & 1u) tmask |= ts.qbit[b];
+    double acc = 0.0;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t dt = tile_base(t, ts) | tmask;
+        dv2 v[TR];
+#pragma unroll
+        for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
+        if (has_x) {
+            __syncthreads();  // the previous tile's partner reads are done
+#pragma unroll
+            for (int r = 0; r < TR; ++r) lds[tid | (r << TB)] = v[r];
+            __syncthreads();
+        }
+        for (uint32_t k = 0; k < ns; ++k) {
+            const SString S = ss[k];
+            double sk = 0.0;
+#pragma unroll
+            for (int r = 0; r < TR; ++r) {
+                bool flip = __popcll(slot_offset(dt, r, ts) & S.z) & 1;
+                double term;
+                if (S.xt == 0) {
+                    term = v[r].x * v[r].x + v[r].y * v[r].y;
+                } else {
+                    const dv2 o = lds[(tid | (r << TB)) ^ S.xt], m = v[r];
+                    const double wr = o.x * m.x + o.y * m.y;
+                    const double wi = o.x * m.y - o.y * m.x;
+                    // Re(i^q (wr + i wi)): wr, -wi, -wr, wi for q = 0..3
+                    const uint32_t q = S.ny & 3u;
+                    term = (q & 1u) ? wi : wr;
+                    flip = flip != (((q + 1u) & 2u) != 0);
+                }
+                sk += flip ? -term : term;
+            }
+            acc += S.coeff * sk;
+        }
+    }
+    const double total = block_sum(acc, ws);
+    if (tid == 0) partial[blockIdx.x] = total;
+}
+
 }  // namespace
+
+// exp_value of a flat state (n >= TL) in grouped sweeps: the strings are put
+// greedily into groups whose X/Y qubits fit one tile (qubits 0..2 plus up to
+// TL - 3 more); every group is one read-only sweep of the state.  Diagonal
+// strings join the first group.
+int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out) {
+    const uint32_t n = s->n;
+    if (s->block || n < (uint32_t)TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "grouped exp_value needs a flat state of at least 11 qubits");
+    const size_t ns = ms.size();
+    std::vector<bool> done(ns, false);
+    struct Group {
+        TileSpec ts;
+        uint32_t first, count, has_x;
+    };
+    std::vector<Group> groups;
+    std::vector<SString> strings;
+    for (;;) {
+        uint64_t qset = 7;
+        bool any = false;
+        for (size_t k = 0; k < ns; ++k) {
+            if (done[k]) continue;
+            any = true;
+            if (__builtin_popcountll(qset | ms[k].x) <= TL) qset |= ms[k].x;
+        }
+        if (!any) break;
+        for (uint32_t q = 3; q < n && __builtin_popcountll(qset) < TL; ++q) qset |= 1ull << q;
+        uint32_t q_at[TL];
+        int b = 0;
+        for (uint32_t q = 0; q < n; ++q)
+            if (qset >> q & 1ull) q_at[b++] = q;
+        Group g;
+        g.ts = make_tile_spec(q_at);
+        g.first = (uint32_t)strings.size();
+        g.has_x = 0;
+        for (size_t k = 0; k < ns; ++k) {
+            if (done[k] || (ms[k].x & ~qset)) continue;
+            SString S{ms[k].z, ms[k].ny, 0, coeffs[k]};
+            for (int p = 0; p < TL; ++p)
+                if (ms[k].x & g.ts.qbit[p]) S.xt |= 1u << p;
+            if (S.xt) g.has_x = 1;
+            strings.push_back(S);
+            done[k] = true;
+        }
+        g.count = (uint32_t)strings.size() - g.first;
+        if (g.count == 0) return set_error(KBWH_ERR_INVALID_ARGUMENT, "Pauli string with more than 8 X/Y qubits above qubit 2");
+        groups.push_back(g);
+    }
+    if (groups.empty()) {
+        *out = 0.0;
+        return KBWH_OK;
+    }
+    if (groups.size() > kResultSlots) return set_error(KBWH_ERR_CAPACITY, "too many string groups");
+    // the string table lives in the tile gate buffer (stream-ordered reuse)
+    const size_t bytes = strings.size() * sizeof(SString);
+    if (s->tile_gates_cap < bytes) {
+        hipError_t e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess && s->tile_gates) e = hipFree(s->tile_gates);
+        s->tile_gates = nullptr;
+        s->tile_gates_cap = 0;
+        if (e == hipSuccess) e = hipMalloc(&s->tile_gates, bytes);
+        if (e != hipSuccess) return hip_status(e, "string table");
+        s->tile_gates_cap = bytes;
+    }
+    hipError_t e = hipMemcpyAsync(s->tile_gates, strings.data(), bytes, hipMemcpyHostToDevice, s->stream);
+    const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
+    const uint32_t grid = std::min<uint32_t>(ntiles, 2048);
+    const SString *ds = (const SString *)s->tile_gates;
+    for (size_t gi = 0; gi < groups.size() && e == hipSuccess; ++gi) {
+        const Group &g = groups[gi];
+        k_pauli_tile<<<grid, TT, 0, s->stream>>>((const dv2 *)s->amp, g.ts, ds + g.first, g.count, g.has_x, ntiles,
+                                                  s->partial);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = launch_sum_final(s->partial, grid, s->result + gi, s->stream);
+    }
+    if (e != hipSuccess) return hip_status(e, "exp_value sweeps");
+    std::vector<double> vals(groups.size());
+    int32_t rc = reduce_to_host(s, vals.data(), (uint32_t)groups.size());
+    if (rc) return rc;
+    double total = 0.0;
+    for (double v : vals) total += v;
+    *out = total;
+    return KBWH_OK;
+}
 
 int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params) {
     return run_tiled_on(s, s->amp, ops, num_ops, params);

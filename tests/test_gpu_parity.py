@@ -258,3 +258,24 @@ def test_full_size_30q_brickwork_prefix_bit_exact():
     # a size-independent property: the norm (<I>) stays 1
     from kbw_amd import Hamiltonian
     assert g.exp_value(Hamiltonian([[]], [1.0])) == pytest.approx(1.0, abs=1e-12)
+
+
+@pytest.mark.parametrize("n,seed", [(11, 0), (16, 1), (20, 2)])
+def test_exp_value_grouped_sweeps(n, seed):
+    """Flat states of 11+ qubits evaluate strings in grouped tile sweeps: many
+    strings with X/Y qubits anywhere, Y factors, a string too wide for one
+    tile (summed alone) and the identity, against the oracle."""
+    from kbw_amd import Hamiltonian
+    rng = np.random.default_rng(seed + 950)
+    ops = circuits.random_circuit(rng, n, 120)
+    ham = circuits.random_hamiltonian(rng, n, 70, max_weight=4)
+    strings = ham.pauli_strings + [[("X", q) for q in range(3, min(n, 13))], [], [("Y", n - 1), ("Y", 0)]]
+    coeffs = ham.coefficients + [0.37, -0.2, 1.1]
+    h = Hamiltonian(strings, coeffs)
+    g, o = pair(n)
+    g.run(ops)
+    o.run(ops)
+    eg = g.exp_value(h)
+    eo = o.exp_value(h.pauli_strings, h.coefficients)
+    assert abs(eg - eo) <= 1e-10 * abs(eo) + 1e-13, (eg, eo)
+    assert g.exp_value(h) == eg  # deterministic

@@ -159,16 +159,23 @@ __global__ __launch_bounds__(kB) void k_adj_param(double2 *__restrict__ phi, dou
 
 inline uint32_t grid_for(uint64_t items, uint64_t per) { return (uint32_t)((items + per - 1) / per); }
 
-int32_t ensure_vec(double2 **p, uint64_t size) {
-    if (*p) return KBWH_OK;
-    return hip_status(hipMalloc(p, size * sizeof(double2)), "adjoint vector");
+// phi and lambda in one allocation, lambda right after phi (the tiled
+// reverse sweep takes the pair as one state of n + 1 qubits)
+int32_t ensure_vecs(kbwh_state *s) {
+    if (s->adj_phi) return KBWH_OK;
+    const int32_t rc = hip_status(hipMalloc(&s->adj_phi, 2 * s->size * sizeof(double2)), "adjoint vectors");
+    if (rc) {
+        s->adj_phi = nullptr;
+        return rc;
+    }
+    s->adj_lam = s->adj_phi + s->size;
+    return KBWH_OK;
 }
 
 }  // namespace
 
 void free_adjoint(kbwh_state *s) {
-    if (s->adj_phi) (void)hipFree(s->adj_phi);
-    if (s->adj_lam) (void)hipFree(s->adj_lam);
+    if (s->adj_phi) (void)hipFree(s->adj_phi);  // lambda shares the allocation
     if (s->adj_partial) (void)hipFree(s->adj_partial);
     s->adj_phi = s->adj_lam = nullptr;
     s->adj_partial = nullptr;
@@ -207,12 +214,13 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
         int32_t rc = pauli_masks(s, paulis, qubits, offsets[k], offsets[k + 1], false, &masks[k]);
         if (rc) return rc;
     }
-    int32_t rc = ensure_vec(&s->adj_phi, size);
-    if (rc == KBWH_OK) rc = ensure_vec(&s->adj_lam, size);
+    int32_t rc = ensure_vecs(s);
     if (rc) return rc;
     uint64_t nparam_ops = 0;
     for (uint64_t i = 0; i < num_ops; ++i) nparam_ops += ops[i].param >= 0;
-    const uint32_t nb = reduce_blocks(size);  // the largest reduction grid below (k_dot_re over all amplitudes)
+    // partial sums: the largest reduction grid below (k_dot_re over all
+    // amplitudes), or the tiled reverse sweep's per-pass partials
+    const uint32_t nb = (uint32_t)std::max<size_t>(reduce_blocks(size), n >= 11 ? kAdjTiledScratch : 0);
     const size_t need = (size_t)nb + nparam_ops + 2;
     if (s->adj_partial_cap < need) {
         if (s->adj_partial) {
@@ -256,9 +264,14 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
         e = hipGetLastError();
     }
     if (e != hipSuccess) return hip_status(e, "adjoint lambda");
-    // reverse sweep
+    // reverse sweep: tiled passes from 11 qubits on, gate by gate below
     uint64_t slot = nparam_ops;
-    for (uint64_t ii = num_ops; ii-- > 0 && e == hipSuccess;) {
+    if (n >= 11) {
+        rc = adjoint_reverse_tiled(s, ops, num_ops, params, slots + 1, partial, nb);
+        if (rc) return rc;
+        slot = 0;
+    }
+    for (uint64_t ii = n >= 11 ? 0 : num_ops; ii-- > 0 && e == hipSuccess;) {
         const kbwh_op &op = ops[ii];
         const double angle = (op.param >= 0) ? params[op.param] : op.angle;
         const uint64_t tbit = 1ull << op.target;

@@ -1,0 +1,286 @@
+// Tiled reverse sweep of the adjoint gradient (states of 11 or more qubits).
+//
+// The reverse sweep un-applies the gates from last to first on phi and lambda
+// and, before un-applying a parameterised gate k, adds its term
+//     RX, RY, RZ:  Im <lambda| Pi_C G |phi>,   G = X, Y, Z on the target
+//     P:           -2 Im <lambda| Pi_{C,t=1} |phi>
+// (see adjoint.hip).  phi and lambda lie back to back in one buffer, which is
+// taken as a state of n + 1 qubits whose top qubit n selects lambda.  Gates
+// of the reversed list then share tiled passes over that state as in
+// KBWH_RUN_TILED, with qubit n pinned to the last register bit of a 12-bit
+// tile: a thread holds amplitude i of phi in slot r and amplitude i of lambda
+// in slot r + 8, every pass reads and writes the two vectors once, and a gate
+// un-applied on both vectors is one uncontrolled gate of the pass.  For RX
+// and RY the term is computed as Im sum_{C(i)} conj((G lambda)_i) phi_i (G is
+// Hermitian and commutes with Pi_C): G controlled on qubit n (so on lambda
+// only) is applied, the masked products taken, and the same gate applied
+// again, which restores lambda (X and Y only exchange and negate parts).
+// Each thread adds its terms to its own LDS slot per parameterised gate of
+// the pass; a workgroup reduces its slots at the end and the workgroup sums
+// are added in a fixed order.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "internal.h"
+#include "state.h"
+// a 12-bit tile: 8 thread bits, register bits 8..10 for targets, 11 for qubit n
+#undef KBWH_TILE_BITS
+#undef KBWH_TILE_TBITS
+#define KBWH_TILE_BITS 12
+#define KBWH_TILE_TBITS 8
+#include "tile_core.h"
+
+#pragma clang fp contract(off)
+
+namespace kbwh {
+namespace {
+
+using namespace tile;
+
+constexpr uint32_t kMaxSlots = 12;  // parameterised gates per pass (LDS accumulators)
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr int kTargets = TL - 4;  // free positions for targets (qubit n takes one)
+
+struct AGate {
+    TGate inv;       // the gate un-applied (the same gate at -angle)
+    TGate gen;       // RX, RY: the generator X or Y on the same tile position, uncontrolled
+    uint32_t slot;   // accumulator of the pass, or kNoSlot
+    uint32_t gkind;  // the parameterised gate's kind
+    uint32_t pad[2];
+};
+
+__device__ __forceinline__ double im_conj_mul(dv2 a, dv2 b) { return a.x * b.y - a.y * b.x; }
+
+__device__ __forceinline__ double block_sum(double v, double *ws) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < TT / 64; ++w) t += ws[w];
+    __syncthreads();
+    return t;
+}
+
+constexpr int kHalf = TR / 2;  // slots r < kHalf hold phi, r + kHalf lambda
+
+// v: the tile of the (n + 1)-qubit state, phi in slots r < kHalf and lambda in r + kHalf
+__global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec ts, const AGate *__restrict__ gs,
+                                                 uint32_t ng, uint32_t ntiles, uint32_t nslots,
+                                                 double *__restrict__ partial) {
+    __shared__ dv2 lds[TT * kChunk];
+    __shared__ double acc[kMaxSlots * TT];
+    __shared__ double ws[TT / 64];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t s = 0; s < nslots; ++s) acc[s * TT + tid] = 0.0;
+    uint64_t tmask = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b)
+        if ((tid >> b) & 1u) tmask |= ts.qbit[b];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t base = tile_base(t, ts);
+        const uint64_t dt = base | tmask;
+        dv2 v[TR];
+#pragma unroll
+        for (int r = 0; r < TR; ++r) v[r] = a[slot_offset(dt, r, ts)];
+        for (uint32_t g = 0; g < ng; ++g) {
+            const AGate &A = gs[g];
+            const TGate G = A.inv;
+            if ((base & G.ocm) != G.ocm) continue;  // a control outside the tile is 0 here
+            if (A.slot != kNoSlot) {
+                const bool tok = thread_ctrl_ok(tid, G.cm);
+                double term = 0.0;
+                if (A.gkind == KRX || A.gkind == KRY) {
+                    const TGate Gg = A.gen;
+                    apply_pair(v, lds, tid, Gg);
+#pragma unroll
+                    for (int r = 0; r < kHalf; ++r)
+                        if (slot_ctrl_ok(tok, r, G.cm)) term += im_conj_mul(v[r + kHalf], v[r]);
+                    apply_pair(v, lds, tid, Gg);
+                } else {
+                    // RZ: +-Im(conj(l) f) by the target bit; P: -2 Im(conj(l) f) on the |1> half
+                    const uint32_t tp = G.tpos;
+                    const bool rz = A.gkind == KRZ;
+#pragma unroll
+                    for (int r = 0; r < kHalf; ++r) {
+                        const bool one = tp == kOutside ? (base & G.otb) != 0
+                                         : tp < (uint32_t)TB ? ((tid >> tp) & 1u) != 0
+                                                             : (((uint32_t)r >> ((tp - TB) & 7u)) & 1u) != 0;
+                        const double w = im_conj_mul(v[r + kHalf], v[r]);
+                        const double c = rz ? (one ? -w : w) : (one ? -2.0 * w : 0.0);
+                        term += slot_ctrl_ok(tok, r, G.cm) ? c : 0.0;
+                    }
+                }
+                acc[A.slot * TT + tid] += term;
+            }
+            if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
+                apply_diag(v, base, tid, G);
+            else
+                apply_pair(v, lds, tid, G);
+        }
+#pragma unroll
+        for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, ts)] = v[r];
+    }
+    for (uint32_t s = 0; s < nslots; ++s) {
+        const double v = block_sum(acc[s * TT + tid], ws);
+        if (tid == 0) partial[(size_t)s * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// out[first_global - s] = sum of the slot's workgroup partials, in order
+__global__ __launch_bounds__(256) void k_sum_slots(const double *__restrict__ partial, uint32_t grid,
+                                                  double *__restrict__ out, uint32_t first_global) {
+    __shared__ double ws[4];
+    const uint32_t s = blockIdx.x;
+    double a = 0.0;
+    for (uint32_t i = threadIdx.x; i < grid; i += 256) a += partial[(size_t)s * grid + i];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) a += __shfl_xor(a, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) out[first_global - s] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+// Tile positions of a pass: qubits 0..2 on lane bits 0..2, qubit n (lambda)
+// on register bit 11, the most used targets on register bits 8..10, then lane
+// bits 3..5, then wave bits 6..7; free positions take the lowest unused qubits.
+void assign_adj(uint32_t n, const Pass &p, int pos_of[64], uint32_t q_at[TL]) {
+    for (int i = 0; i < 64; ++i) pos_of[i] = -1;
+    for (int b = 0; b < TL; ++b) q_at[b] = 0xFFFFFFFFu;
+    for (uint32_t q = 0; q < 3; ++q) {
+        q_at[q] = q;
+        pos_of[q] = (int)q;
+    }
+    q_at[TL - 1] = n;
+    pos_of[n] = TL - 1;
+    std::vector<std::pair<int, uint32_t>> freq;
+    for (uint32_t t : p.targets) {
+        if (t < 3) continue;
+        int f = 0;
+        for (const PhysOp &o : p.ops)
+            if (!is_diagonal(o.kind) && o.target == t) ++f;
+        freq.push_back({-f, t});
+    }
+    std::stable_sort(freq.begin(), freq.end(),
+                     [](const std::pair<int, uint32_t> &x, const std::pair<int, uint32_t> &y) {
+                         return x.first < y.first;
+                     });
+    const int order[kTargets] = {TB, TB + 1, TB + 2, 3, 4, 5, 6, 7};
+    for (size_t k = 0; k < freq.size(); ++k) {
+        q_at[order[k]] = freq[k].second;
+        pos_of[freq[k].second] = order[k];
+    }
+    uint32_t filler = 3;
+    for (int b = 3; b < TL - 1; ++b) {
+        if (q_at[b] != 0xFFFFFFFFu) continue;
+        while (pos_of[filler] >= 0) ++filler;
+        q_at[b] = filler;
+        pos_of[filler] = b;
+        ++filler;
+    }
+}
+
+}  // namespace
+
+// The reverse sweep over phi and lambda (s->adj_phi, lambda right after it)
+// in tiled passes; slots[j] receives the term of the j-th parameterised op
+// (forward order).  Needs n >= 11.
+int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                              double *slots, double *partial, size_t partial_cap) {
+    const uint32_t n = s->n;
+    if (n < 11 || n > 62) return set_error(KBWH_ERR_INVALID_ARGUMENT, "tiled reverse sweep needs 11..62 qubits");
+    uint64_t nparam = 0;
+    for (uint64_t i = 0; i < num_ops; ++i) nparam += ops[i].param >= 0;
+    struct RPass {
+        TileSpec ts;
+        uint32_t first, count, nslots, first_global;
+    };
+    std::vector<RPass> passes;
+    std::vector<AGate> table;
+    Pass cur;
+    std::vector<uint32_t> cur_slot, cur_gkind;
+    uint32_t cur_params = 0, cur_first_global = 0;
+    uint64_t global = nparam;  // the reverse walk meets parameterised ops from the last one
+    auto close = [&]() {
+        if (cur.ops.empty()) return;
+        int pos_of[64];
+        uint32_t q_at[TL];
+        assign_adj(n, cur, pos_of, q_at);
+        RPass P;
+        P.ts = make_tile_spec(q_at);
+        P.first = (uint32_t)table.size();
+        P.count = (uint32_t)cur.ops.size();
+        P.nslots = cur_params;
+        P.first_global = cur_first_global;
+        for (size_t i = 0; i < cur.ops.size(); ++i) {
+            AGate A;
+            memset(&A, 0, sizeof A);
+            A.inv = make_tgate(cur.ops[i], pos_of, n + 1);
+            A.slot = cur_slot[i];
+            A.gkind = cur_gkind[i];
+            if (A.slot != kNoSlot && (A.gkind == KRX || A.gkind == KRY)) {
+                // the generator on lambda only: controlled on qubit n
+                const PhysOp gen{A.gkind == KRX ? (int)KX : (int)KY, cur.ops[i].target, 1ull << n, 0.0};
+                A.gen = make_tgate(gen, pos_of, n + 1);
+            }
+            table.push_back(A);
+        }
+        passes.push_back(P);
+        cur = Pass();
+        cur_slot.clear();
+        cur_gkind.clear();
+        cur_params = 0;
+    };
+    for (uint64_t ii = num_ops; ii-- > 0;) {
+        const kbwh_op &op = ops[ii];
+        const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
+        const bool rot = op.kind == KRX || op.kind == KRY || op.kind == KRZ || op.kind == KP;
+        const PhysOp po{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), rot ? -angle : angle};
+        const bool param = op.param >= 0;
+        if (!fits(cur, po, kTargets) || (param && cur_params == kMaxSlots)) close();
+        add_op(cur, po);
+        if (param) {
+            --global;
+            if (cur_params == 0) cur_first_global = (uint32_t)global;
+            cur_slot.push_back(cur_params++);
+        } else {
+            cur_slot.push_back(kNoSlot);
+        }
+        cur_gkind.push_back(op.kind);
+    }
+    close();
+    const uint32_t ntiles = (uint32_t)(1ull << (n - 11));
+    const uint32_t grid = std::min<uint32_t>(ntiles, kAdjTiledGrid);
+    if ((size_t)grid * kMaxSlots > partial_cap) return set_error(KBWH_ERR_DEVICE, "adjoint scratch too small");
+    const size_t bytes = table.size() * sizeof(AGate);
+    if (s->tile_gates_cap < bytes) {
+        hipError_t e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess && s->tile_gates) e = hipFree(s->tile_gates);
+        s->tile_gates = nullptr;
+        s->tile_gates_cap = 0;
+        if (e == hipSuccess) e = hipMalloc(&s->tile_gates, bytes);
+        if (e != hipSuccess) return hip_status(e, "adjoint gate table");
+        s->tile_gates_cap = bytes;
+    }
+    hipError_t e = bytes ? hipMemcpyAsync(s->tile_gates, table.data(), bytes, hipMemcpyHostToDevice, s->stream)
+                         : hipSuccess;
+    const AGate *dg = (const AGate *)s->tile_gates;
+    for (const RPass &P : passes) {
+        if (e != hipSuccess) break;
+        k_adj_tile<<<grid, TT, 0, s->stream>>>((dv2 *)s->adj_phi, P.ts, dg + P.first, P.count, ntiles, P.nslots,
+                                               partial);
+        e = hipGetLastError();
+        if (e == hipSuccess && P.nslots) {
+            k_sum_slots<<<P.nslots, 256, 0, s->stream>>>(partial, grid, slots, P.first_global);
+            e = hipGetLastError();
+        }
+    }
+    // the host gate table is local: the upload must be done before it goes away
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return hip_status(e, "adjoint reverse passes");
+}
+
+}  // namespace kbwh

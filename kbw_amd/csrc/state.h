@@ -65,6 +65,12 @@ int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const dou
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
                      const double *params);
 void free_adjoint(kbwh_state *s);
+// Tiled reverse sweep of the adjoint gradient (n >= 11): slots[j] receives the
+// term of the j-th parameterised op; partial holds kAdjTiledScratch doubles.
+constexpr uint32_t kAdjTiledGrid = 1024;
+constexpr size_t kAdjTiledScratch = (size_t)kAdjTiledGrid * 12;
+int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                              double *slots, double *partial, size_t partial_cap);
 int32_t queue_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint64_t ctrl_mask);
 int32_t flush_queue(kbwh_state *s);
 int32_t run_fused(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops, const double *params,

@@ -45,10 +45,14 @@ def test_adjoint_matches_oracle_parameter_shift(n, layers, ham_kind):
     assert st[0] == 1 and np.count_nonzero(st) == 1
 
 
-def test_adjoint_controlled_and_phase_vs_finite_differences():
-    n = 9
-    rng = np.random.default_rng(4)
-    ops = circuits.random_circuit(rng, n, 60, max_controls=2)
+@pytest.mark.parametrize("n,num_ops", [(9, 60), (11, 90), (12, 160)])
+def test_adjoint_controlled_and_phase_vs_finite_differences(n, num_ops):
+    """Controlled RX/RY/RZ/P and fixed gates; from 11 qubits the reverse sweep
+    runs in tiled passes (controls inside and outside the tile, more
+    parameterised gates than one pass takes)."""
+    print(f"adjoint fd n={n}", flush=True)
+    rng = np.random.default_rng(4 + n)
+    ops = circuits.random_circuit(rng, n, num_ops, max_controls=2)
     kinds = ops["kind"]
     par = np.isin(kinds, [4, 5, 6, 7])
     ops["param"][par] = np.arange(par.sum())

@@ -252,10 +252,32 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
             e = hipGetLastError();
         }
     }
-    for (uint32_t k = 0; k < num_strings && e == hipSuccess; ++k) {
-        k_pauli_apply<<<g1, kB, 0, st>>>(s->adj_phi, s->adj_lam, size, masks[k].x, masks[k].z,
-                                         masks[k].ny, coeffs[k], k == 0);
+    // from 11 qubits, the strings whose X/Y qubits fit one tile (qubits 0..2
+    // and 8 more) in grouped sweeps, one read of phi per group; the others
+    // one launch each
+    std::vector<PauliMasks> narrow;
+    std::vector<double> narrow_coeffs;
+    std::vector<uint32_t> wide;
+    for (uint32_t k = 0; k < num_strings; ++k) {
+        if (n >= 11 && __builtin_popcountll(masks[k].x | 7ull) <= 11) {
+            narrow.push_back(masks[k]);
+            narrow_coeffs.push_back(coeffs[k]);
+        } else {
+            wide.push_back(k);
+        }
+    }
+    bool assigned = false;
+    if (!narrow.empty()) {
+        rc = pauli_apply_tiled(s, s->adj_phi, s->adj_lam, narrow, narrow_coeffs.data());
+        if (rc) return rc;
+        assigned = true;
+    }
+    for (size_t w = 0; w < wide.size() && e == hipSuccess; ++w) {
+        const uint32_t k = wide[w];
+        k_pauli_apply<<<g1, kB, 0, st>>>(s->adj_phi, s->adj_lam, size, masks[k].x, masks[k].z, masks[k].ny,
+                                         coeffs[k], !assigned);
         e = hipGetLastError();
+        assigned = true;
     }
     if (e == hipSuccess) {
         const uint32_t gd = reduce_blocks(size);

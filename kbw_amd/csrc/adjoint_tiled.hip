@@ -40,13 +40,12 @@ namespace {
 
 using namespace tile;
 
-constexpr uint32_t kMaxSlots = 12;  // parameterised gates per pass (LDS accumulators)
+constexpr uint32_t kMaxSlots = kAdjTiledSlots;  // LDS accumulators: TT doubles each
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr int kTargets = TL - 4;  // free positions for targets (qubit n takes one)
 
 struct AGate {
     TGate inv;       // the gate un-applied (the same gate at -angle)
-    TGate gen;       // RX, RY: the generator X or Y on the same tile position, uncontrolled
     uint32_t slot;   // accumulator of the pass, or kNoSlot
     uint32_t gkind;  // the parameterised gate's kind
     uint32_t pad[2];
@@ -66,6 +65,46 @@ __device__ __forceinline__ double block_sum(double v, double *ws) {
 }
 
 constexpr int kHalf = TR / 2;  // slots r < kHalf hold phi, r + kHalf lambda
+
+// o[r]: the phi amplitude paired with slot r (r < kHalf) across register bit RB
+template <int RB>
+__device__ __forceinline__ void reg_partners(const dv2 (&v)[TR], dv2 (&o)[kHalf]) {
+#pragma unroll
+    for (int r = 0; r < kHalf; ++r) o[r] = v[r ^ (1 << RB)];
+}
+
+// o[r]: the phi amplitude paired with slot r across tile position tp (a
+// target, so never the lambda bit).  Register bits are dispatched on their
+// bits (no multi-way branch on tp), lane bits go through shuffles and wave
+// bits through LDS, kChunk slots at a time.
+__device__ __forceinline__ void phi_partners(const dv2 (&v)[TR], dv2 (&o)[kHalf], dv2 *lds, uint32_t tid,
+                                             uint32_t tp) {
+    if (tp >= (uint32_t)TB) {
+        const uint32_t rb = tp - TB;
+        if (rb & 2u)
+            reg_partners<2>(v, o);
+        else if (rb & 1u)
+            reg_partners<1>(v, o);
+        else
+            reg_partners<0>(v, o);
+        return;
+    }
+    if (tp < 6) {
+#pragma unroll
+        for (int r = 0; r < kHalf; ++r) o[r] = shfl_xor2(v[r], 1 << tp);
+        return;
+    }
+    const uint32_t pt = tid ^ (1u << tp);
+#pragma unroll
+    for (int q = 0; q < kHalf; q += kChunk) {
+        __syncthreads();  // earlier exchange reads are done
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) lds[tid + (r << TB)] = v[q + r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) o[q + r] = lds[pt + (r << TB)];
+    }
+}
 
 // v: the tile of the (n + 1)-qubit state, phi in slots r < kHalf and lambda in r + kHalf
 __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec ts, const AGate *__restrict__ gs,
@@ -92,23 +131,30 @@ __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec t
             if ((base & G.ocm) != G.ocm) continue;  // a control outside the tile is 0 here
             if (A.slot != kNoSlot) {
                 const bool tok = thread_ctrl_ok(tid, G.cm);
+                const uint32_t tp = G.tpos;
+                // the target bit of slot r: fixed per tile (outside), per thread (lane/wave) or per slot
+                const bool reg = tp - (uint32_t)TB < 8u;
+                const uint32_t rb = (tp - TB) & 7u;
+                const bool one_t = tp == kOutside ? (base & G.otb) != 0 : ((tid >> (tp & 31u)) & 1u) != 0;
                 double term = 0.0;
                 if (A.gkind == KRX || A.gkind == KRY) {
-                    const TGate Gg = A.gen;
-                    apply_pair(v, lds, tid, Gg);
+                    // Im conj(lambda_i) (G phi)_i, G = X or Y: (X phi)_i = phi_p, (Y phi)_i = -+i phi_p
+                    dv2 o[kHalf];
+                    phi_partners(v, o, lds, tid, tp);
+                    const bool y = A.gkind == KRY;
 #pragma unroll
-                    for (int r = 0; r < kHalf; ++r)
-                        if (slot_ctrl_ok(tok, r, G.cm)) term += im_conj_mul(v[r + kHalf], v[r]);
-                    apply_pair(v, lds, tid, Gg);
+                    for (int r = 0; r < kHalf; ++r) {
+                        const bool one = reg ? (((uint32_t)r >> rb) & 1u) != 0 : one_t;
+                        const dv2 q = !y ? o[r] : one ? mk2(-o[r].y, o[r].x) : mk2(o[r].y, -o[r].x);
+                        const double w = im_conj_mul(v[r + kHalf], q);
+                        term += slot_ctrl_ok(tok, r, G.cm) ? w : 0.0;
+                    }
                 } else {
                     // RZ: +-Im(conj(l) f) by the target bit; P: -2 Im(conj(l) f) on the |1> half
-                    const uint32_t tp = G.tpos;
                     const bool rz = A.gkind == KRZ;
 #pragma unroll
                     for (int r = 0; r < kHalf; ++r) {
-                        const bool one = tp == kOutside ? (base & G.otb) != 0
-                                         : tp < (uint32_t)TB ? ((tid >> tp) & 1u) != 0
-                                                             : (((uint32_t)r >> ((tp - TB) & 7u)) & 1u) != 0;
+                        const bool one = reg ? (((uint32_t)r >> rb) & 1u) != 0 : one_t;
                         const double w = im_conj_mul(v[r + kHalf], v[r]);
                         const double c = rz ? (one ? -w : w) : (one ? -2.0 * w : 0.0);
                         term += slot_ctrl_ok(tok, r, G.cm) ? c : 0.0;
@@ -221,11 +267,6 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
             A.inv = make_tgate(cur.ops[i], pos_of, n + 1);
             A.slot = cur_slot[i];
             A.gkind = cur_gkind[i];
-            if (A.slot != kNoSlot && (A.gkind == KRX || A.gkind == KRY)) {
-                // the generator on lambda only: controlled on qubit n
-                const PhysOp gen{A.gkind == KRX ? (int)KX : (int)KY, cur.ops[i].target, 1ull << n, 0.0};
-                A.gen = make_tgate(gen, pos_of, n + 1);
-            }
             table.push_back(A);
         }
         passes.push_back(P);

@@ -68,7 +68,8 @@ void free_adjoint(kbwh_state *s);
 // Tiled reverse sweep of the adjoint gradient (n >= 11): slots[j] receives the
 // term of the j-th parameterised op; partial holds kAdjTiledScratch doubles.
 constexpr uint32_t kAdjTiledGrid = 1024;
-constexpr size_t kAdjTiledScratch = (size_t)kAdjTiledGrid * 12;
+constexpr uint32_t kAdjTiledSlots = 24;  // parameterised gates per pass
+constexpr size_t kAdjTiledScratch = (size_t)kAdjTiledGrid * kAdjTiledSlots;
 int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
                               double *slots, double *partial, size_t partial_cap);
 int32_t queue_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint64_t ctrl_mask);
@@ -82,6 +83,8 @@ struct PauliMasks {
     uint32_t ny;
 };
 int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out);
+int32_t pauli_apply_tiled(kbwh_state *s, const double2 *phi, double2 *lam, const std::vector<PauliMasks> &ms,
+                          const double *coeffs);
 int32_t pauli_masks(const kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
                     uint32_t lo, uint32_t hi, bool physical, PauliMasks *out);
 

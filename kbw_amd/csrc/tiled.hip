@@ -145,23 +145,69 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     if (tid == 0) partial[blockIdx.x] = total;
 }
 
-}  // namespace
+// lambda (+)= sum_k coeff_k P_k phi over the strings of one group, whose X/Y
+// qubits are all tile qubits of ts; one tile per workgroup.  phi's tile is
+// read once (partners through LDS) and lambda's tile read (assign = 0) and
+// written once.  Per string and amplitude the arithmetic is k_pauli_apply's:
+// (P phi)_i = i^q phi_{i^x}, q = ny + 2 popcount((i ^ x) & z), times coeff,
+// added to lambda_i; here ss[k].ny holds ny + 2 popcount(x & z) (mod 4), so
+// q = ny + 2 popcount(i & z).
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_pauli_apply_tile(
+    const dv2 *__restrict__ phi, dv2 *__restrict__ lam, TileSpec ts, const SString *__restrict__ ss, uint32_t ns,
+    uint32_t has_x, uint32_t ntiles, uint32_t assign) {
+    __shared__ dv2 lds[TT * TR];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    uint64_t tmask = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b)
+        if ((tid >> b) & 1u) tmask |= ts.qbit[b];
+    const uint64_t dt = tile_base(tile, ts) | tmask;
+    dv2 v[TR], out[TR];
+#pragma unroll
+    for (int r = 0; r < TR; ++r) v[r] = phi[slot_offset(dt, r, ts)];
+    if (has_x) {
+#pragma unroll
+        for (int r = 0; r < TR; ++r) lds[tid | (r << TB)] = v[r];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < TR; ++r) out[r] = assign ? mk2(0.0, 0.0) : lam[slot_offset(dt, r, ts)];
+    for (uint32_t k = 0; k < ns; ++k) {
+        const SString S = ss[k];
+        // popcount(i & z) = popcount(dt & z) + popcount(r & zr), zr = z on the register bits
+        uint32_t zr = 0;
+#pragma unroll
+        for (int b = 0; b < TL - TB; ++b)
+            if (ts.qbit[TB + b] & S.z) zr |= 1u << b;
+        const uint32_t qd = S.ny + 2u * (uint32_t)(__popcll(dt & S.z) & 1);
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+            const dv2 f = S.xt ? lds[(tid | (r << TB)) ^ S.xt] : v[r];
+            const uint32_t q = (qd + 2u * (uint32_t)(__builtin_popcount((uint32_t)r & zr) & 1)) & 3u;
+            // i^q f: (f.x, f.y), (-f.y, f.x), (-f.x, -f.y), (f.y, -f.x) for q = 0..3
+            const double a = (q & 1u) ? f.y : f.x, b = (q & 1u) ? f.x : f.y;
+            const double re = ((q + 1u) & 2u) ? -a : a, im = (q & 2u) ? -b : b;
+            out[r] = mk2(out[r].x + re * S.coeff, out[r].y + im * S.coeff);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < TR; ++r) lam[slot_offset(dt, r, ts)] = out[r];
+}
 
-// exp_value of a flat state (n >= TL) in grouped sweeps: the strings are put
-// greedily into groups whose X/Y qubits fit one tile (qubits 0..2 plus up to
-// TL - 3 more); every group is one read-only sweep of the state.  Diagonal
-// strings join the first group.
-int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out) {
-    const uint32_t n = s->n;
-    if (s->block || n < (uint32_t)TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "grouped exp_value needs a flat state of at least 11 qubits");
+struct SGroup {
+    TileSpec ts;
+    uint32_t first, count, has_x;
+};
+
+// Strings put greedily into groups whose X/Y qubits fit one tile (qubits 0..2
+// plus up to TL - 3 more); diagonal strings join the first group.  With
+// apply_form, ny also takes the sign of popcount(x & z) (k_pauli_apply_tile).
+int32_t group_strings(uint32_t n, const std::vector<PauliMasks> &ms, const double *coeffs, bool apply_form,
+                      std::vector<SGroup> &groups, std::vector<SString> &strings) {
     const size_t ns = ms.size();
     std::vector<bool> done(ns, false);
-    struct Group {
-        TileSpec ts;
-        uint32_t first, count, has_x;
-    };
-    std::vector<Group> groups;
-    std::vector<SString> strings;
     for (;;) {
         uint64_t qset = 7;
         bool any = false;
@@ -176,13 +222,15 @@ int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const 
         int b = 0;
         for (uint32_t q = 0; q < n; ++q)
             if (qset >> q & 1ull) q_at[b++] = q;
-        Group g;
+        SGroup g;
         g.ts = make_tile_spec(q_at);
         g.first = (uint32_t)strings.size();
         g.has_x = 0;
         for (size_t k = 0; k < ns; ++k) {
             if (done[k] || (ms[k].x & ~qset)) continue;
-            SString S{ms[k].z, ms[k].ny, 0, coeffs[k]};
+            uint32_t ny = ms[k].ny;
+            if (apply_form) ny = (ny + 2u * (uint32_t)(__builtin_popcountll(ms[k].x & ms[k].z) & 1)) & 3u;
+            SString S{ms[k].z, ny, 0, coeffs[k]};
             for (int p = 0; p < TL; ++p)
                 if (ms[k].x & g.ts.qbit[p]) S.xt |= 1u << p;
             if (S.xt) g.has_x = 1;
@@ -193,12 +241,11 @@ int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const 
         if (g.count == 0) return set_error(KBWH_ERR_INVALID_ARGUMENT, "Pauli string with more than 8 X/Y qubits above qubit 2");
         groups.push_back(g);
     }
-    if (groups.empty()) {
-        *out = 0.0;
-        return KBWH_OK;
-    }
-    if (groups.size() > kResultSlots) return set_error(KBWH_ERR_CAPACITY, "too many string groups");
-    // the string table lives in the tile gate buffer (stream-ordered reuse)
+    return KBWH_OK;
+}
+
+// the string table into the tile gate buffer (stream-ordered reuse)
+int32_t upload_strings(kbwh_state *s, const std::vector<SString> &strings) {
     const size_t bytes = strings.size() * sizeof(SString);
     if (s->tile_gates_cap < bytes) {
         hipError_t e = hipStreamSynchronize(s->stream);
@@ -209,12 +256,37 @@ int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const 
         if (e != hipSuccess) return hip_status(e, "string table");
         s->tile_gates_cap = bytes;
     }
-    hipError_t e = hipMemcpyAsync(s->tile_gates, strings.data(), bytes, hipMemcpyHostToDevice, s->stream);
+    if (bytes == 0) return KBWH_OK;
+    return hip_status(hipMemcpyAsync(s->tile_gates, strings.data(), bytes, hipMemcpyHostToDevice, s->stream),
+                      "string table");
+}
+
+}  // namespace
+
+// exp_value of a flat state (n >= TL) in grouped sweeps: the strings are put
+// greedily into groups whose X/Y qubits fit one tile (qubits 0..2 plus up to
+// TL - 3 more); every group is one read-only sweep of the state.  Diagonal
+// strings join the first group.
+int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out) {
+    const uint32_t n = s->n;
+    if (s->block || n < (uint32_t)TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "grouped exp_value needs a flat state of at least 11 qubits");
+    std::vector<SGroup> groups;
+    std::vector<SString> strings;
+    int32_t rc = group_strings(n, ms, coeffs, false, groups, strings);
+    if (rc) return rc;
+    if (groups.empty()) {
+        *out = 0.0;
+        return KBWH_OK;
+    }
+    if (groups.size() > kResultSlots) return set_error(KBWH_ERR_CAPACITY, "too many string groups");
+    rc = upload_strings(s, strings);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
     const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
     const uint32_t grid = std::min<uint32_t>(ntiles, 2048);
     const SString *ds = (const SString *)s->tile_gates;
     for (size_t gi = 0; gi < groups.size() && e == hipSuccess; ++gi) {
-        const Group &g = groups[gi];
+        const SGroup &g = groups[gi];
         k_pauli_tile<<<grid, TT, 0, s->stream>>>((const dv2 *)s->amp, g.ts, ds + g.first, g.count, g.has_x, ntiles,
                                                   s->partial);
         e = hipGetLastError();
@@ -222,12 +294,39 @@ int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const 
     }
     if (e != hipSuccess) return hip_status(e, "exp_value sweeps");
     std::vector<double> vals(groups.size());
-    int32_t rc = reduce_to_host(s, vals.data(), (uint32_t)groups.size());
+    rc = reduce_to_host(s, vals.data(), (uint32_t)groups.size());
     if (rc) return rc;
     double total = 0.0;
     for (double v : vals) total += v;
     *out = total;
     return KBWH_OK;
+}
+
+// lam = sum_k coeffs[k] P_k phi for vectors of s->n >= TL qubits, in grouped
+// sweeps (one read of phi and one read-modify-write of lam per group).
+int32_t pauli_apply_tiled(kbwh_state *s, const double2 *phi, double2 *lam, const std::vector<PauliMasks> &ms,
+                          const double *coeffs) {
+    const uint32_t n = s->n;
+    if (n < (uint32_t)TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "grouped Pauli sums need at least 11 qubits");
+    std::vector<SGroup> groups;
+    std::vector<SString> strings;
+    int32_t rc = group_strings(n, ms, coeffs, true, groups, strings);
+    if (rc) return rc;
+    if (groups.empty()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "no Pauli strings");
+    rc = upload_strings(s, strings);
+    if (rc) return rc;
+    const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
+    const SString *ds = (const SString *)s->tile_gates;
+    hipError_t e = hipSuccess;
+    for (size_t gi = 0; gi < groups.size() && e == hipSuccess; ++gi) {
+        const SGroup &g = groups[gi];
+        k_pauli_apply_tile<<<ntiles, TT, 0, s->stream>>>((const dv2 *)phi, (dv2 *)lam, g.ts, ds + g.first, g.count,
+                                                          g.has_x, ntiles, gi == 0);
+        e = hipGetLastError();
+    }
+    // the host string table is local: the upload must be done before it goes away
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return hip_status(e, "Pauli sum sweeps");
 }
 
 int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params) {

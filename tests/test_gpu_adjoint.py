@@ -72,6 +72,30 @@ def test_adjoint_controlled_and_phase_vs_finite_differences(n, num_ops):
     assert np.max(np.abs(g - fd)) < 1e-7
 
 
+def test_adjoint_grouped_and_wide_strings():
+    """From 11 qubits lambda = H phi is built in grouped tile sweeps; a string
+    with more than 8 X/Y qubits above qubit 2 takes a launch of its own."""
+    from kbw_amd.dense import Hamiltonian
+
+    n = 13
+    print(f"adjoint strings n={n}", flush=True)
+    ops, params = circuits.hea(n, 2, seed=5)
+    strings = [
+        [("X", q) for q in range(3, 13)] + [("Y", 0)],  # 10 X/Y qubits above qubit 2
+        [("Y", 4)],
+        [("Y", 2), ("Z", 7)],
+        [("X", 1), ("Y", 11), ("Z", 12)],
+        [("Z", 0), ("Z", 12)],
+        [("Y", q) for q in range(5, 14 - 1)],  # 8 Y qubits above qubit 2: fits one tile
+    ]
+    ham = Hamiltonian(strings, [0.7, -0.4, 1.3, 0.25, -0.9, 0.55])
+    sim = DenseHip(n)
+    e, g = sim.adjoint_gradient(ops, params, ham)
+    assert e == pytest.approx(oracle_energy(n, ops, params, ham), abs=1e-12)
+    ref = oracle_shift(n, ops, params, ham)
+    assert np.max(np.abs(g - ref)) < 1e-10
+
+
 def test_adjoint_24q_subset_vs_plain_parameter_shift():
     """The 24-qubit, 12-layer, 576-parameter ansatz with the XXZ chain: the
     adjoint gradient against the GPU plain path's shift rule on 6 parameters."""

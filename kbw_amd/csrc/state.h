@@ -57,6 +57,8 @@ int32_t set_error(int32_t code, const std::string &msg);
 int32_t hip_status(hipError_t e, const char *what);
 int32_t ensure_scratch(kbwh_state *s, size_t doubles);
 int32_t local_qubit(kbwh_state *s, uint32_t logical, uint32_t *out);
+constexpr uint32_t kNoSwap = 0xFFFFFFFFu;
+void map_qubit(kbwh_state *s, uint32_t logical, uint32_t *out, uint32_t *swap_local, uint32_t *swap_global);
 uint64_t physical_mask(const kbwh_state *s, uint64_t logical_mask);
 int32_t apply_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint64_t ctrl_mask);
 int32_t reduce_to_host(kbwh_state *s, double *out, uint32_t count);

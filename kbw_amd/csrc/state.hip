@@ -79,22 +79,24 @@ static void touch(kbwh_state *s, uint32_t phys) {
 }
 
 // block.rs:123-156: a global target is swapped with the least recently used
-// local qubit (first minimum, block.rs:111-117).
-int32_t local_qubit(kbwh_state *s, uint32_t logical, uint32_t *out) {
+// local qubit (first minimum, block.rs:111-117).  Only the maps change here;
+// *swap_local / *swap_global name the physical bits whose amplitudes the
+// caller must exchange (kNoSwap when the target is local).
+void map_qubit(kbwh_state *s, uint32_t logical, uint32_t *out, uint32_t *swap_local, uint32_t *swap_global) {
+    *swap_local = *swap_global = kNoSwap;
     if (!s->block) {
         *out = logical;
-        return KBWH_OK;
+        return;
     }
     const uint32_t phys = s->l2p[logical];
     if (phys < s->nlocal) {
         touch(s, phys);
         *out = phys;
-        return KBWH_OK;
+        return;
     }
     uint32_t lq = 0;
     for (uint32_t i = 1; i < s->nlocal; ++i)
         if (s->last_access[i] < s->last_access[lq]) lq = i;
-    HIPCHK(launch_swap_bits(s->amp, s->n, lq, phys, s->stream));
     s->swaps++;
     const uint32_t old_logical = s->p2l[lq];
     s->l2p[logical] = lq;
@@ -103,6 +105,14 @@ int32_t local_qubit(kbwh_state *s, uint32_t logical, uint32_t *out) {
     s->p2l[phys] = old_logical;
     touch(s, lq);
     *out = lq;
+    *swap_local = lq;
+    *swap_global = phys;
+}
+
+int32_t local_qubit(kbwh_state *s, uint32_t logical, uint32_t *out) {
+    uint32_t lq, g;
+    map_qubit(s, logical, out, &lq, &g);
+    if (lq != kNoSwap) HIPCHK(launch_swap_bits(s->amp, s->n, lq, g, s->stream));
     return KBWH_OK;
 }
 

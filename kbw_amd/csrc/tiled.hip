@@ -35,8 +35,13 @@ using namespace tile;
 
 // One tile per workgroup: load the tile into registers, apply the gates,
 // store it back; overlap comes from the other resident workgroups.
+// PERM: Block swaps folded into the pass; the tile is stored through st, whose
+// qubits are those of ts in another order (position b of the tile goes to
+// qubit st.qbit[b]), so exchanged qubits cost no extra traffic.
+template <bool PERM>
 __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_tile(
-    dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles) {
+    dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles,
+    TileSpec st) {
     __shared__ dv2 lds[TT * kChunk];
     const uint32_t tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
@@ -67,10 +72,25 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     } while (++i < ng);
     // recompute the store addresses instead of keeping eight of them live
     // through the gate loop
-    uint64_t ds = dt;
-    asm volatile("" : "+v"(ds));
+    if constexpr (PERM) {
+        // the tile goes to addresses that other waves of the workgroup
+        // loaded: every load must have returned before any store is issued
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        uint64_t sm = 0;
 #pragma unroll
-    for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, ts));
+        for (int b = 0; b < TB; ++b)
+            if ((tid >> b) & 1u) sm |= st.qbit[b];
+        uint64_t ds = base | sm;
+        asm volatile("" : "+v"(ds));
+#pragma unroll
+        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, st));
+    } else {
+        uint64_t ds = dt;
+        asm volatile("" : "+v"(ds));
+#pragma unroll
+        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, ts));
+    }
 }
 
 
@@ -356,25 +376,59 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         if (n < 64 && (op.ctrl_mask >> n)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "control out of range");
     }
     Pass cur;
-    // Execute: build passes lazily, flushing before any Block swap.
+    // Block swaps are folded into the passes.  A pass is planned in the
+    // qubits of the state as its tiles are loaded; ld[p] is the load-time
+    // qubit whose amplitudes now sit at physical qubit p (after the swaps
+    // made so far in the pass) and ph is its inverse.  Each swap must keep
+    // both of its qubits in the tile; the pass stores the tile through the
+    // permuted map.
+    std::vector<uint32_t> ld(n), ph(n);
+    for (uint32_t q = 0; q < n; ++q) ld[q] = ph[q] = q;
+    bool swapped = false;
     std::vector<TGate> gates;
     struct Launch {
-        TileSpec spec;
+        TileSpec spec, store;
         uint32_t first, count;
+        bool perm;
     };
     std::vector<Launch> launches;
+    const uint32_t cap = std::min<uint32_t>(s->tile_targets, TL - 3);
     auto flush = [&]() -> int32_t {
-        if (cur.ops.empty()) return KBWH_OK;
+        if (cur.ops.empty() && !swapped) return KBWH_OK;
         int pos_of[64];
         uint32_t q_at[TL];
         assign_positions(n, cur, pos_of, q_at);
         Launch L;
         L.spec = make_tile_spec(q_at);
+        L.store = L.spec;
+        L.perm = swapped;
+        if (swapped) {
+            uint32_t q_st[TL];
+            uint64_t in = 0, out = 0;
+            for (int b = 0; b < TL; ++b) {
+                q_st[b] = ph[q_at[b]];
+                in |= 1ull << q_at[b];
+                out |= 1ull << q_st[b];
+            }
+            if (in != out) return set_error(KBWH_ERR_DEVICE, "swapped qubit outside the tile");
+            L.store = make_tile_spec(q_st);
+        }
         L.first = (uint32_t)gates.size();
         for (const PhysOp &o : cur.ops) gates.push_back(make_tgate(o, pos_of, n));
-        L.count = (uint32_t)cur.ops.size();
+        if (cur.ops.empty()) {
+            // swaps only: one record that no tile applies (its control is
+            // qubit 0, always a tile qubit, so never set in a tile base)
+            TGate g;
+            memset(&g, 0, sizeof g);
+            g.kind = KZ;
+            g.ocm = 1;
+            gates.push_back(g);
+        }
+        L.count = (uint32_t)gates.size() - L.first;
         launches.push_back(L);
         cur = Pass();
+        for (uint32_t q = 0; q < n; ++q) ld[q] = ph[q] = q;
+        swapped = false;
         return KBWH_OK;
     };
     auto launch_all = [&]() -> int32_t {
@@ -396,7 +450,10 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         const uint32_t grid = ntiles;
         const TGate *dg = (const TGate *)s->tile_gates;
         for (const Launch &L : launches) {
-            k_tile<<<grid, TT, 0, s->stream>>>((dv2 *)amp, L.spec, dg + L.first, L.count, ntiles);
+            if (L.perm)
+                k_tile<true><<<grid, TT, 0, s->stream>>>((dv2 *)amp, L.spec, dg + L.first, L.count, ntiles, L.store);
+            else
+                k_tile<false><<<grid, TT, 0, s->stream>>>((dv2 *)amp, L.spec, dg + L.first, L.count, ntiles, L.spec);
             e = hipGetLastError();
             if (e != hipSuccess) return hip_status(e, "k_tile launch");
         }
@@ -406,21 +463,54 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         gates.clear();
         return KBWH_OK;
     };
+    // tile positions a swap of physical qubits a and b still needs in the pending pass
+    auto swap_needs = [&](uint32_t a, uint32_t b) -> uint32_t {
+        uint32_t need = 0;
+        for (uint32_t q : {ld[a], ld[b]})
+            if (q >= 3 && std::find(cur.targets.begin(), cur.targets.end(), q) == cur.targets.end()) ++need;
+        return need;
+    };
+    auto require = [&](uint32_t q) {
+        if (q >= 3 && std::find(cur.targets.begin(), cur.targets.end(), q) == cur.targets.end())
+            cur.targets.push_back(q);
+    };
     for (uint64_t i = 0; i < num_ops; ++i) {
         const kbwh_op &op = ops[i];
         const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
-        uint32_t t = op.target;
-        if (s->block && s->l2p[op.target] >= s->nlocal) {
-            // a global target: finish the pending passes, then swap (block.rs:132-150)
-            int32_t rc = flush();
-            if (rc == KBWH_OK) rc = launch_all();
+        uint32_t t = op.target, sl = kNoSwap, sg = kNoSwap;
+        if (s->block) map_qubit(s, op.target, &t, &sl, &sg);
+        if (sl != kNoSwap) {
+            // a global target exchanged with a local qubit (block.rs:132-150)
+            int32_t rc = KBWH_OK;
+            if (cur.targets.size() + swap_needs(sl, sg) > cap) rc = flush();
             if (rc) return rc;
+            if (swap_needs(sl, sg) > cap) {
+                // cannot share a tile under this target cap: a swap pass of its own
+                rc = launch_all();
+                if (rc) return rc;
+                hipError_t e = launch_swap_bits(amp, n, sl, sg, s->stream);
+                if (e != hipSuccess) return hip_status(e, "swap launch");
+            } else {
+                require(ld[sl]);
+                require(ld[sg]);
+                std::swap(ld[sl], ld[sg]);
+                ph[ld[sl]] = sl;
+                ph[ld[sg]] = sg;
+                swapped = true;
+            }
         }
-        int32_t rc = local_qubit(s, op.target, &t);
-        if (rc) return rc;
-        uint64_t m = physical_mask(s, op.ctrl_mask) & ~(1ull << t);
-        const PhysOp po{(int)op.kind, t, m, angle};
-        if (!fits(cur, po, s->tile_targets) && (rc = flush())) return rc;
+        const uint64_t m = physical_mask(s, op.ctrl_mask) & ~(1ull << t);
+        auto load_time = [&]() {
+            uint64_t lm = 0;
+            for (uint64_t r = m; r; r &= r - 1) lm |= 1ull << ld[__builtin_ctzll(r)];
+            return PhysOp{(int)op.kind, ld[t], lm, angle};
+        };
+        PhysOp po = load_time();
+        if (!fits(cur, po, cap)) {
+            int32_t rc = flush();
+            if (rc) return rc;
+            po = load_time();
+        }
         add_op(cur, po);
     }
     int32_t rc = flush();

@@ -108,6 +108,28 @@ def test_tiled_block_mode_bit_exact():
     assert_same(g.state(), o.physical_state())
 
 
+@pytest.mark.parametrize("targets", [1, 2, 3, 8])
+def test_tiled_block_swaps_in_passes(targets):
+    """Block swaps folded into tiled passes (the tile stored through the
+    exchanged qubits): same map, swap count and physical amplitudes as the
+    oracle's Block.  With one target position per pass a swap cannot share a
+    tile and takes a swap pass of its own."""
+    n, L = 15, 11
+    print(f"block swaps targets={targets}", flush=True)
+    rng = np.random.default_rng(17)
+    ops = circuits.random_circuit(rng, n, 300, max_controls=3)
+    g = DenseHip(n, block=True, block_size=L)
+    g.set_run_config(tile_targets=targets)
+    g.run(ops, mode=RUN_TILED)
+    o = oracle.Block(n, L)
+    for op in ops:
+        m = int(op["ctrl_mask"])
+        o.gate(int(op["kind"]), float(op["angle"]), int(op["target"]), [q for q in range(n) if m >> q & 1])
+    assert np.array_equal(g.qubit_map(), o.qubit_map())
+    assert g.swap_count() == o.swap_count() > 20
+    assert_same(g.state(), o.physical_state())
+
+
 def test_tiled_params_vector():
     n = 14
     ops, params = circuits.hea(n, 3, seed=4)

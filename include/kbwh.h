@@ -67,6 +67,8 @@ extern "C" {
 #define KBWH_RUN_TILED 1u    /* gates grouped into multi-gate passes over LDS tiles; same per-pair arithmetic, same result bits */
 #define KBWH_RUN_FUSED 2u    /* gates fused into k-qubit blocks applied with FP64 MFMA; equal to rounding (~1e-16 per block), not bit for bit */
 #define KBWH_RUN_AUTO 3u     /* the configuration chosen by kbwh_autotune for this handle (KBWH_RUN_TILED until tuned) */
+#define KBWH_RUN_REORDER 4u  /* tiled passes over a dependency order of the gates (gates on disjoint qubits may trade places):
+                                the same unitary, amplitudes equal to rounding; fewer passes.  Block states run as KBWH_RUN_TILED */
 
 /* One circuit operation (24 bytes).  `angle` is used unless `param` >= 0
  * and a parameter vector is supplied, in which case the angle is
@@ -113,18 +115,26 @@ int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const doub
  * (default 64: as many as the tile's free positions, 8 for 11-qubit tiles). */
 int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_targets);
 
+/* The pass plan of KBWH_RUN_REORDER for a flat state of num_qubits qubits,
+ * on the host only (no device needed; no counterpart in the reference):
+ * order[0..num_ops) receives the op indices in execution order and
+ * pass_end[0..*num_passes) the end of each pass in `order` (pass_end needs
+ * room for num_ops entries).  tile_targets as in kbwh_set_run_config. */
+int32_t kbwh_reorder_plan(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits, uint32_t tile_targets,
+                          uint32_t *order, uint32_t *pass_end, uint64_t *num_passes);
+
 /* Autotuner (no counterpart in the reference): times each candidate
  * configuration of `ops` (at most the first 256 gates) on a scratch buffer
  * the size of the state, leaving the state untouched, and keeps the fastest
  * for KBWH_RUN_AUTO.  Candidates: per-gate; tiled with all free positions,
  * 6 and 4 targets per pass; with allow_inexact, fused blocks of 4 and 5
- * qubits.  ms[i] is the
+ * qubits and dependency-ordered tiled passes.  ms[i] is the
  * time of candidate i in that order (0 when not run).  Needs device memory
  * for a second state (KBWH_ERR_OUT_OF_MEMORY otherwise). */
 typedef struct kbwh_tune_result {
-    uint32_t mode;          /* KBWH_RUN_PER_GATE, KBWH_RUN_TILED or KBWH_RUN_FUSED */
+    uint32_t mode;          /* KBWH_RUN_PER_GATE, KBWH_RUN_TILED, KBWH_RUN_FUSED or KBWH_RUN_REORDER */
     uint32_t fusion_width;  /* for KBWH_RUN_FUSED */
-    uint32_t tile_targets;  /* for KBWH_RUN_TILED (64: no cap) */
+    uint32_t tile_targets;  /* for KBWH_RUN_TILED and KBWH_RUN_REORDER (64: no cap) */
     uint32_t num_candidates;
     double ms[8];
 } kbwh_tune_result;

@@ -5,11 +5,11 @@ in include/kbwh.h); this package is its host-side mirror of the reference's
 QuantumExecution interface.
 """
 from ._lib import (H, OP_DTYPE, P, PAULI_X, PAULI_Y, PAULI_Z, RUN_AUTO, RUN_FUSED, RUN_PER_GATE,
-                   RUN_TILED, RX, RY, RZ, X, Y, Z, KbwhError, build_info, load)
+                   RUN_REORDER, RUN_TILED, RX, RY, RZ, X, Y, Z, KbwhError, build_info, load)
 from .dense import DenseHip, DumpData, Hamiltonian, QubitManager, ops_array
 
 __all__ = [
     "DenseHip", "DumpData", "Hamiltonian", "QubitManager", "KbwhError", "OP_DTYPE", "ops_array",
     "build_info", "load", "X", "Y", "Z", "H", "P", "RX", "RY", "RZ", "PAULI_X", "PAULI_Y",
-    "PAULI_Z", "RUN_PER_GATE", "RUN_TILED", "RUN_FUSED", "RUN_AUTO",
+    "PAULI_Z", "RUN_PER_GATE", "RUN_TILED", "RUN_FUSED", "RUN_AUTO", "RUN_REORDER",
 ]

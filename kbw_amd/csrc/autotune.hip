@@ -26,6 +26,7 @@ struct Candidate {
 const Candidate kCandidates[] = {
     {KBWH_RUN_PER_GATE, 5, 64, true}, {KBWH_RUN_TILED, 5, 64, true}, {KBWH_RUN_TILED, 5, 6, true},
     {KBWH_RUN_TILED, 5, 4, true},     {KBWH_RUN_FUSED, 4, 64, false}, {KBWH_RUN_FUSED, 5, 64, false},
+    {KBWH_RUN_REORDER, 5, 64, false},
 };
 
 int32_t run_on(kbwh_state *s, double2 *buf, const Candidate &c, const kbwh_op *ops, uint64_t n,
@@ -39,10 +40,11 @@ int32_t run_on(kbwh_state *s, double2 *buf, const Candidate &c, const kbwh_op *o
             if (e != hipSuccess) return hip_status(e, "gate launch");
         }
         return KBWH_OK;
-    case KBWH_RUN_TILED: {
+    case KBWH_RUN_TILED:
+    case KBWH_RUN_REORDER: {
         const uint32_t keep = s->tile_targets;
         s->tile_targets = c.tile_targets;
-        const int32_t rc = run_tiled_on(s, buf, ops, n, params);
+        const int32_t rc = run_tiled_on(s, buf, ops, n, params, c.mode == KBWH_RUN_REORDER);
         s->tile_targets = keep;
         return rc;
     }
@@ -129,7 +131,7 @@ int32_t kbwh_autotune(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const
     if (rc) return rc;
     s->auto_mode = out->mode;
     if (out->mode == KBWH_RUN_FUSED) s->fusion_width = out->fusion_width;
-    if (out->mode == KBWH_RUN_TILED) s->tile_targets = out->tile_targets;
+    if (out->mode == KBWH_RUN_TILED || out->mode == KBWH_RUN_REORDER) s->tile_targets = out->tile_targets;
     return KBWH_OK;
 }
 

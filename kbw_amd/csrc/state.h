@@ -64,8 +64,9 @@ int32_t apply_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint6
 int32_t reduce_to_host(kbwh_state *s, double *out, uint32_t count);
 double uniform_scale(double total);
 int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params);
+// reorder: dependency-ordered passes (KBWH_RUN_REORDER; flat states only)
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
-                     const double *params);
+                     const double *params, bool reorder = false);
 void free_adjoint(kbwh_state *s);
 // Tiled reverse sweep of the adjoint gradient (n >= 11): slots[j] receives the
 // term of the j-th parameterised op; partial holds kAdjTiledScratch doubles.

@@ -119,18 +119,19 @@ __device__ __forceinline__ dv2 pair_out(dv2 m, dv2 o, const PairCoef &k) {
     return mk2(((k.A * m.x + k.B * o.x) + k.Cr * o.y) * k.D, ((k.A * m.y + k.B * o.y) + k.Ci * o.x) * k.D);
 }
 
-// Uncontrolled Y, H, RX and RY in two terms instead of three:
-//     out = (A m + B q) D,   q = o or q = (o.im, -o.re)  (rotated: Y, RX)
-// with A, D as in the generic form and B = Cr for the rotated kinds:
-//     Y:  A=0    B=-sigma  rotated       (|0>: (o.im, -o.re), |1>: (-o.im, o.re))
-//     H:  A=-sigma B=1     D=1/sqrt2
-//     RX: A=c    B=s       rotated       (c m.re + s o.im, c m.im - s o.re)
-//     RY: A=c    B=sigma*s
-// 8 operations per amplitude instead of 12, each output equal in value to
-// the generic form's.  The kernels keep exactly two uncontrolled pair code
-// paths (X and this one): gfx950 code built with a separate path per kind
-// gave wrong amplitudes for every kind, X included.
-enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2 };
+// Uncontrolled gates take shorter forms, each equal in value to the generic
+// form above (and to the reference's closure):
+//     X:  out = o                                   (exchange)
+//     Y:  |0>: (o.im, -o.re), |1>: (-o.im, o.re)    (exchange and sign flips)
+//     H:  |0>: (m + o) k,     |1>: (o - m) k         (k = 1/sqrt2)
+//     RX: (c m.re + s o.im, c m.im - s o.re)
+//     RY: c m + sigma s o
+// RX and RY share one path (PK_TWO): out = A m + B q with q = o, or
+// q = (o.im, -o.re) for RX (B = Cr), and no product by D = 1.  Controlled
+// gates other than X use the generic form (PK_GEN).  Every path is chosen by
+// uniform two-way tests on the gate record, never by a multi-way branch (see
+// apply_pair_t).
+enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2, PK_Y = 3, PK_H = 4 };
 
 // Control test for the amplitude in register slot r of thread tid, whose tile
 // index is tid + TT r: the thread bits are tested once per gate (tok) and the
@@ -148,12 +149,18 @@ template <int PK>
 __device__ __forceinline__ dv2 pair_val(dv2 m, dv2 o, bool one, const TGate &G) {
     if constexpr (PK == PK_X) {
         return o;
+    } else if constexpr (PK == PK_Y) {
+        return one ? mk2(-o.y, o.x) : mk2(o.y, -o.x);
+    } else if constexpr (PK == PK_H) {
+        const double sx = one ? o.x - m.x : m.x + o.x;
+        const double sy = one ? o.y - m.y : m.y + o.y;
+        return mk2(sx * G.d, sy * G.d);
     } else if constexpr (PK == PK_TWO) {
         const PairCoef k = pair_coef(G, one ? 1.0 : -1.0);
-        const bool rot = G.kind == KRX || G.kind == KY;
+        const bool rot = G.kind == KRX;
         const double b = rot ? k.Cr : k.B;
         const double qx = rot ? o.y : o.x, qy = rot ? -o.x : o.y;
-        return mk2((k.A * m.x + b * qx) * k.D, (k.A * m.y + b * qy) * k.D);
+        return mk2(k.A * m.x + b * qx, k.A * m.y + b * qy);
     } else {
         return pair_out(m, o, pair_coef(G, one ? 1.0 : -1.0));
     }
@@ -269,6 +276,10 @@ __device__ __forceinline__ void apply_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid,
     }
     if (G.cm != 0)
         apply_pair_t<PK_GEN, true>(v, lds, tid, G);
+    else if (G.kind == KY)
+        apply_pair_t<PK_Y, false>(v, lds, tid, G);
+    else if (G.kind == KH)
+        apply_pair_t<PK_H, false>(v, lds, tid, G);
     else
         apply_pair_t<PK_TWO, false>(v, lds, tid, G);
 }

@@ -349,12 +349,136 @@ int32_t pauli_apply_tiled(kbwh_state *s, const double2 *phi, double2 *lam, const
     return hip_status(e, "Pauli sum sweeps");
 }
 
+namespace {
+
+// Dependency-ordered pass planning (KBWH_RUN_REORDER).  Gates that share no
+// qubit (target or control) commute, so any order that keeps, on every qubit,
+// the order of the gates touching it is the same unitary; only the rounding
+// of the amplitudes can change.  A gate is ready when it is the first pending
+// gate on each of its qubits.  A pass starts with tile qubits 0..2 and runs
+// every ready gate it can (diagonal, or target in the tile), again and again;
+// while target positions are left, the target of a ready gate that lets the
+// most pending gates run (the lowest such qubit on ties) joins the tile.
+// order: op indices in execution order; ends[p]: end of pass p in order.
+void schedule_passes(const std::vector<PhysOp> &ops, uint32_t n, uint32_t free_targets,
+                     std::vector<uint32_t> &order, std::vector<uint32_t> &ends) {
+    const size_t m = ops.size();
+    std::vector<uint64_t> qm(m);
+    std::vector<uint32_t> start(n + 1, 0);
+    for (size_t i = 0; i < m; ++i) {
+        qm[i] = ops[i].ctrl | (1ull << ops[i].target);
+        for (uint64_t r = qm[i]; r; r &= r - 1) ++start[__builtin_ctzll(r) + 1];
+    }
+    for (uint32_t q = 0; q < n; ++q) start[q + 1] += start[q];
+    std::vector<uint32_t> list(start[n]);
+    {
+        std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+        for (size_t i = 0; i < m; ++i)
+            for (uint64_t r = qm[i]; r; r &= r - 1) list[fill[__builtin_ctzll(r)]++] = (uint32_t)i;
+    }
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    auto front = [&](uint32_t q, const std::vector<uint32_t> &h) {
+        return h[q] < start[q + 1] ? list[h[q]] : kNone;
+    };
+    auto ready = [&](uint32_t i, const std::vector<uint32_t> &h) {
+        for (uint64_t r = qm[i]; r; r &= r - 1)
+            if (front(__builtin_ctzll(r), h) != i) return false;
+        return true;
+    };
+    auto allowed = [&](uint32_t i, uint64_t tile) {
+        return is_diagonal(ops[i].kind) || ((tile >> ops[i].target) & 1ull);
+    };
+    std::vector<uint32_t> work;
+    // run every gate that can run in `tile`; appends them to out when given
+    auto closure = [&](std::vector<uint32_t> &h, uint64_t tile, std::vector<uint32_t> *out) {
+        size_t count = 0;
+        work.clear();
+        for (uint32_t q = 0; q < n; ++q) work.push_back(q);
+        while (!work.empty()) {
+            const uint32_t q = work.back();
+            work.pop_back();
+            const uint32_t i = front(q, h);
+            if (i == kNone || !ready(i, h) || !allowed(i, tile)) continue;
+            for (uint64_t r = qm[i]; r; r &= r - 1) {
+                const uint32_t p = __builtin_ctzll(r);
+                ++h[p];
+                work.push_back(p);
+            }
+            ++count;
+            if (out) out->push_back(i);
+        }
+        return count;
+    };
+    std::vector<uint32_t> head(start.begin(), start.end() - 1), trial;
+    size_t left = m;
+    order.clear();
+    ends.clear();
+    while (left) {
+        uint64_t tile = 7;
+        left -= closure(head, tile, &order);
+        for (uint32_t used = 0; used < free_targets && left; ++used) {
+            uint64_t cands = 0;
+            for (uint32_t q = 0; q < n; ++q) {
+                const uint32_t i = front(q, head);
+                if (i != kNone && !is_diagonal(ops[i].kind) && !((tile >> ops[i].target) & 1ull) && ready(i, head))
+                    cands |= 1ull << ops[i].target;
+            }
+            if (!cands) break;
+            uint32_t best = 0;
+            size_t best_count = 0;
+            for (uint64_t r = cands; r; r &= r - 1) {
+                const uint32_t t = __builtin_ctzll(r);
+                trial = head;
+                const size_t c = closure(trial, tile | (1ull << t), nullptr);
+                if (c > best_count) {
+                    best_count = c;
+                    best = t;
+                }
+            }
+            tile |= 1ull << best;
+            left -= closure(head, tile, &order);
+        }
+        ends.push_back((uint32_t)order.size());
+    }
+}
+
+}  // namespace
+
+}  // namespace kbwh
+
+// The dependency-ordered pass plan of KBWH_RUN_REORDER on the host only.
+extern "C" int32_t kbwh_reorder_plan(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits,
+                                     uint32_t tile_targets, uint32_t *order, uint32_t *pass_end,
+                                     uint64_t *num_passes) {
+    using namespace kbwh;
+    if ((num_ops && (!ops || !order || !pass_end)) || !num_passes)
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_qubits < 3 || num_qubits > 64) return set_error(KBWH_ERR_INVALID_ARGUMENT, "3..64 qubits");
+    if (tile_targets < 1 || tile_targets > 64) return set_error(KBWH_ERR_INVALID_ARGUMENT, "tile targets must be 1..64");
+    if (num_ops > 0xFFFFFFFFull) return set_error(KBWH_ERR_INVALID_ARGUMENT, "too many ops");
+    std::vector<PhysOp> phys(num_ops);
+    for (uint64_t i = 0; i < num_ops; ++i) {
+        const kbwh_op &op = ops[i];
+        if (op.kind > KRZ || op.target >= num_qubits || (num_qubits < 64 && (op.ctrl_mask >> num_qubits)))
+            return set_error(KBWH_ERR_INVALID_ARGUMENT, "op out of range");
+        phys[i] = PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), op.angle};
+    }
+    std::vector<uint32_t> ord, ends;
+    schedule_passes(phys, num_qubits, std::min<uint32_t>(tile_targets, TL - 3), ord, ends);
+    std::copy(ord.begin(), ord.end(), order);
+    std::copy(ends.begin(), ends.end(), pass_end);
+    *num_passes = ends.size();
+    return KBWH_OK;
+}
+
+namespace kbwh {
+
 int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params) {
-    return run_tiled_on(s, s->amp, ops, num_ops, params);
+    return run_tiled_on(s, s->amp, ops, num_ops, params, false);
 }
 
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
-                     const double *params) {
+                     const double *params, bool reorder) {
     if (s->n < TL) {
         // the whole state is smaller than one tile: one pass per gate
         if (amp == s->amp) return kbwh_run(s, ops, num_ops, params, KBWH_RUN_PER_GATE);
@@ -474,6 +598,27 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         if (q >= 3 && std::find(cur.targets.begin(), cur.targets.end(), q) == cur.targets.end())
             cur.targets.push_back(q);
     };
+    if (reorder && !s->block) {
+        // dependency-ordered passes (a Block state keeps program order: its
+        // swaps follow the order in which targets are met)
+        std::vector<PhysOp> phys(num_ops);
+        for (uint64_t i = 0; i < num_ops; ++i) {
+            const kbwh_op &op = ops[i];
+            const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
+            phys[i] = PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), angle};
+        }
+        std::vector<uint32_t> order, ends;
+        schedule_passes(phys, n, cap, order, ends);
+        size_t k = 0;
+        for (const uint32_t end : ends) {
+            for (; k < end; ++k) add_op(cur, phys[order[k]]);
+            const int32_t rc = flush();
+            if (rc) return rc;
+        }
+        const int32_t rc = launch_all();
+        s->last_gates = num_ops;
+        return rc;
+    }
     for (uint64_t i = 0; i < num_ops; ++i) {
         const kbwh_op &op = ops[i];
         const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;

@@ -72,6 +72,21 @@ def ops_array(ops):
     return a
 
 
+def reorder_plan(ops, num_qubits, tile_targets=64):
+    """The pass plan of RUN_REORDER (host only): (order, passes), where
+    order lists the op indices in execution order and passes[p] is the
+    (start, end) range of pass p in order."""
+    a = ops_array(ops)
+    order = np.zeros(len(a), dtype=np.uint32)
+    ends = np.zeros(max(len(a), 1), dtype=np.uint32)
+    npass = C.c_uint64()
+    check(_lib.load().kbwh_reorder_plan(ptr(a), len(a), num_qubits, tile_targets, ptr(order), ptr(ends),
+                                        C.byref(npass)))
+    ends = ends[:npass.value].astype(np.int64)
+    starts = np.concatenate([[0], ends[:-1]])
+    return order.astype(np.int64), list(zip(starts.tolist(), ends.tolist()))
+
+
 class DenseHip:
     """QuantumExecution on one MI355X: the dense c128 state lives in HBM."""
 

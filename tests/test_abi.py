@@ -44,6 +44,7 @@ def test_constants_match_header():
     assert int(consts["KBWH_PAULI_Y"]) == _lib.PAULI_Y
     assert int(consts["KBWH_RUN_TILED"]) == _lib.RUN_TILED
     assert int(consts["KBWH_RUN_FUSED"]) == _lib.RUN_FUSED and int(consts["KBWH_RUN_AUTO"]) == _lib.RUN_AUTO
+    assert int(consts["KBWH_RUN_REORDER"]) == _lib.RUN_REORDER
 
 
 def test_tune_result_layout_matches_header():

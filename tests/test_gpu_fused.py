@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from kbw_amd import RUN_AUTO, RUN_FUSED, RUN_PER_GATE, RUN_TILED, DenseHip, circuits
+from kbw_amd import RUN_AUTO, RUN_FUSED, RUN_PER_GATE, RUN_REORDER, RUN_TILED, DenseHip, circuits
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -89,17 +89,18 @@ def test_autotune_picks_a_measured_configuration(allow_inexact):
     mode, k, targets, ms = g.autotune(ops, allow_inexact=allow_inexact)
     print(f"autotune inexact={allow_inexact}: mode {mode} k {k} targets {targets} ms {ms}", flush=True)
     assert np.array_equal(g.state(), before)  # the state is not touched
-    assert len(ms) == 6
+    # per-gate, tiled (3 target caps); inexact: fused 4 and 5, reordered
+    assert len(ms) == 7
     run = [m for i, m in enumerate(ms) if allow_inexact or i < 4]
     assert all(m > 0 for m in run)
     if not allow_inexact:
-        assert mode in (RUN_PER_GATE, RUN_TILED) and ms[4] == 0 and ms[5] == 0
+        assert mode in (RUN_PER_GATE, RUN_TILED) and ms[4] == ms[5] == ms[6] == 0
     # RUN_AUTO uses the choice; exact choices reproduce the oracle bit for bit
     g.clear()
     g.run(ops, mode=RUN_AUTO)
     o = oracle.DenseV2(n)
     o.run(ops)
-    if mode == RUN_FUSED:
+    if mode in (RUN_FUSED, RUN_REORDER):
         assert float(np.max(np.abs(g.state() - o.state()))) < TOL
     else:
         assert np.array_equal(g.state(), o.state())

@@ -241,7 +241,7 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
     // forward: phi = U |0>, with tiled passes on the phi buffer
     hipError_t e = launch_init(s->adj_phi, size, st);
     if (e != hipSuccess) return hip_status(e, "adjoint init");
-    rc = run_tiled_on(s, s->adj_phi, ops, num_ops, params);
+    rc = run_tiled_on(s, s->adj_phi, ops, num_ops, params, true);  // dependency-ordered passes
     if (rc) return rc;
     // lambda = H phi and E = Re <phi|lambda>
     const uint32_t g1 = grid_for(size, kB);

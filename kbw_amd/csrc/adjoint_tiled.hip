@@ -6,18 +6,19 @@
 //     P:           -2 Im <lambda| Pi_{C,t=1} |phi>
 // (see adjoint.hip).  phi and lambda lie back to back in one buffer, which is
 // taken as a state of n + 1 qubits whose top qubit n selects lambda.  Gates
-// of the reversed list then share tiled passes over that state as in
-// KBWH_RUN_TILED, with qubit n pinned to the last register bit of a 12-bit
-// tile: a thread holds amplitude i of phi in slot r and amplitude i of lambda
-// in slot r + 8, every pass reads and writes the two vectors once, and a gate
-// un-applied on both vectors is one uncontrolled gate of the pass.  For RX
-// and RY the term is computed as Im sum_{C(i)} conj((G lambda)_i) phi_i (G is
-// Hermitian and commutes with Pi_C): G controlled on qubit n (so on lambda
-// only) is applied, the masked products taken, and the same gate applied
-// again, which restores lambda (X and Y only exchange and negate parts).
-// Each thread adds its terms to its own LDS slot per parameterised gate of
-// the pass; a workgroup reduces its slots at the end and the workgroup sums
-// are added in a fixed order.
+// of the reversed list share tiled passes over that state, planned in
+// dependency order as in KBWH_RUN_REORDER (a term is unchanged when both
+// vectors first go through gates on other qubits, which commute with G),
+// with qubit n pinned to the last register bit of a 12-bit tile: a thread
+// holds amplitude i of phi in slot r and amplitude i of lambda in slot r + 8,
+// every pass reads and writes the two vectors once, and a gate un-applied on
+// both vectors is one uncontrolled gate of the pass.  For RX and RY the term
+// is Im sum_{C(i)} conj(lambda_i) (G phi)_i, where (G phi)_i is the phi
+// amplitude paired with i across the target (times -+i for Y), fetched like
+// a gate's partner (registers, lane shuffles or LDS).  Each thread adds its
+// terms to its own LDS slot per parameterised gate of the pass; a workgroup
+// reduces its slots at the end and the workgroup sums are added in a fixed
+// order.
 #include <math.h>
 #include <string.h>
 
@@ -43,6 +44,11 @@ using namespace tile;
 constexpr uint32_t kMaxSlots = kAdjTiledSlots;  // LDS accumulators: TT doubles each
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr int kTargets = TL - 4;  // free positions for targets (qubit n takes one)
+
+// the parameter index (among the parameterised ops) of each slot of a pass
+struct SlotMap {
+    uint32_t global[kMaxSlots];
+};
 
 struct AGate {
     TGate inv;       // the gate un-applied (the same gate at -angle)
@@ -176,9 +182,9 @@ __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec t
     }
 }
 
-// out[first_global - s] = sum of the slot's workgroup partials, in order
+// out[map.global[s]] = the sum of slot s's workgroup partials, in order
 __global__ __launch_bounds__(256) void k_sum_slots(const double *__restrict__ partial, uint32_t grid,
-                                                  double *__restrict__ out, uint32_t first_global) {
+                                                  double *__restrict__ out, SlotMap map) {
     __shared__ double ws[4];
     const uint32_t s = blockIdx.x;
     double a = 0.0;
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(256) void k_sum_slots(const double *__restrict__ pa
     for (int off = 32; off >= 1; off >>= 1) a += __shfl_xor(a, off, 64);
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = a;
     __syncthreads();
-    if (threadIdx.x == 0) out[first_global - s] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+    if (threadIdx.x == 0) out[map.global[s]] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 // Tile positions of a pass: qubits 0..2 on lane bits 0..2, qubit n (lambda)
@@ -242,57 +248,63 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
     for (uint64_t i = 0; i < num_ops; ++i) nparam += ops[i].param >= 0;
     struct RPass {
         TileSpec ts;
-        uint32_t first, count, nslots, first_global;
+        uint32_t first, count, nslots;
+        SlotMap map;
     };
+    // the reversed list (each gate at -angle for the rotations); rev[j] is
+    // ops[num_ops - 1 - j], param_of[j] its index among the parameterised ops
+    std::vector<PhysOp> rev(num_ops);
+    std::vector<uint8_t> is_param(num_ops);
+    std::vector<uint32_t> param_of(num_ops, kNoSlot);
+    {
+        uint64_t global = nparam;
+        for (uint64_t j = 0; j < num_ops; ++j) {
+            const kbwh_op &op = ops[num_ops - 1 - j];
+            const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
+            const bool rot = op.kind == KRX || op.kind == KRY || op.kind == KRZ || op.kind == KP;
+            rev[j] = PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), rot ? -angle : angle};
+            is_param[j] = op.param >= 0;
+            if (is_param[j]) param_of[j] = (uint32_t)--global;
+        }
+    }
+    // dependency-ordered passes: a gradient term is Im <lambda| G |phi>
+    // with G on the gate's own qubits, unchanged when both vectors first go
+    // through gates on other qubits (they commute with G), so the reversed
+    // list may be scheduled like KBWH_RUN_REORDER, kMaxSlots terms per pass
+    std::vector<uint32_t> order, ends;
+    schedule_passes(rev, n, kTargets, order, ends, &is_param, kMaxSlots);
     std::vector<RPass> passes;
     std::vector<AGate> table;
-    Pass cur;
-    std::vector<uint32_t> cur_slot, cur_gkind;
-    uint32_t cur_params = 0, cur_first_global = 0;
-    uint64_t global = nparam;  // the reverse walk meets parameterised ops from the last one
-    auto close = [&]() {
-        if (cur.ops.empty()) return;
+    size_t k = 0;
+    for (const uint32_t end : ends) {
+        Pass cur;
+        for (size_t i = k; i < end; ++i) add_op(cur, rev[order[i]]);
         int pos_of[64];
         uint32_t q_at[TL];
         assign_adj(n, cur, pos_of, q_at);
         RPass P;
+        memset(&P.map, 0, sizeof P.map);
         P.ts = make_tile_spec(q_at);
         P.first = (uint32_t)table.size();
-        P.count = (uint32_t)cur.ops.size();
-        P.nslots = cur_params;
-        P.first_global = cur_first_global;
-        for (size_t i = 0; i < cur.ops.size(); ++i) {
+        P.count = (uint32_t)(end - k);
+        P.nslots = 0;
+        for (size_t i = k; i < end; ++i) {
+            const uint32_t j = order[i];
             AGate A;
             memset(&A, 0, sizeof A);
-            A.inv = make_tgate(cur.ops[i], pos_of, n + 1);
-            A.slot = cur_slot[i];
-            A.gkind = cur_gkind[i];
+            A.inv = make_tgate(rev[j], pos_of, n + 1);
+            A.gkind = (uint32_t)rev[j].kind;
+            A.slot = kNoSlot;
+            if (is_param[j]) {
+                A.slot = P.nslots;
+                P.map.global[P.nslots++] = param_of[j];
+            }
             table.push_back(A);
         }
+        if (P.nslots > kMaxSlots) return set_error(KBWH_ERR_DEVICE, "adjoint pass over its slots");
         passes.push_back(P);
-        cur = Pass();
-        cur_slot.clear();
-        cur_gkind.clear();
-        cur_params = 0;
-    };
-    for (uint64_t ii = num_ops; ii-- > 0;) {
-        const kbwh_op &op = ops[ii];
-        const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
-        const bool rot = op.kind == KRX || op.kind == KRY || op.kind == KRZ || op.kind == KP;
-        const PhysOp po{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), rot ? -angle : angle};
-        const bool param = op.param >= 0;
-        if (!fits(cur, po, kTargets) || (param && cur_params == kMaxSlots)) close();
-        add_op(cur, po);
-        if (param) {
-            --global;
-            if (cur_params == 0) cur_first_global = (uint32_t)global;
-            cur_slot.push_back(cur_params++);
-        } else {
-            cur_slot.push_back(kNoSlot);
-        }
-        cur_gkind.push_back(op.kind);
+        k = end;
     }
-    close();
     const uint32_t ntiles = (uint32_t)(1ull << (n - 11));
     const uint32_t grid = std::min<uint32_t>(ntiles, kAdjTiledGrid);
     if ((size_t)grid * kMaxSlots > partial_cap) return set_error(KBWH_ERR_DEVICE, "adjoint scratch too small");
@@ -315,7 +327,7 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
                                                partial);
         e = hipGetLastError();
         if (e == hipSuccess && P.nslots) {
-            k_sum_slots<<<P.nslots, 256, 0, s->stream>>>(partial, grid, slots, P.first_global);
+            k_sum_slots<<<P.nslots, 256, 0, s->stream>>>(partial, grid, slots, P.map);
             e = hipGetLastError();
         }
     }

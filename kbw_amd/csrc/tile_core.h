@@ -12,7 +12,15 @@
 
 #include "state.h"
 
+// Exact gate arithmetic rounds every product and sum on its own, as the
+// reference does.  A unit that defines KBWH_TILE_CONTRACT (the dependency-
+// ordered runs, equal to the reference only to rounding anyway) lets the
+// compiler fuse a product into the following add.
+#ifdef KBWH_TILE_CONTRACT
+#pragma clang fp contract(fast)
+#else
 #pragma clang fp contract(off)
+#endif
 
 namespace kbwh {
 namespace tile {
@@ -496,6 +504,107 @@ inline void add_op(Pass &p, const PhysOp &o) {
         std::find(p.targets.begin(), p.targets.end(), o.target) == p.targets.end())
         p.targets.push_back(o.target);
     p.ops.push_back(o);
+}
+
+// Dependency-ordered pass planning (KBWH_RUN_REORDER).  Gates that share no
+// qubit (target or control) commute, so any order that keeps, on every qubit,
+// the order of the gates touching it is the same unitary; only the rounding
+// of the amplitudes can change.  A gate is ready when it is the first pending
+// gate on each of its qubits.  A pass starts with tile qubits 0..2 and runs
+// every ready gate it can (diagonal, or target in the tile), again and again;
+// while target positions are left, the target of a ready gate that lets the
+// most pending gates run (the lowest such qubit on ties) joins the tile.
+// order: op indices in execution order; ends[p]: end of pass p in order.
+// limited (optional): ops of which a pass may hold at most max_limited.
+inline void schedule_passes(const std::vector<PhysOp> &ops, uint32_t n, uint32_t free_targets,
+                            std::vector<uint32_t> &order, std::vector<uint32_t> &ends,
+                            const std::vector<uint8_t> *limited = nullptr, uint32_t max_limited = 0) {
+    const size_t m = ops.size();
+    std::vector<uint64_t> qm(m);
+    std::vector<uint32_t> start(n + 1, 0);
+    for (size_t i = 0; i < m; ++i) {
+        qm[i] = ops[i].ctrl | (1ull << ops[i].target);
+        for (uint64_t r = qm[i]; r; r &= r - 1) ++start[__builtin_ctzll(r) + 1];
+    }
+    for (uint32_t q = 0; q < n; ++q) start[q + 1] += start[q];
+    std::vector<uint32_t> list(start[n]);
+    {
+        std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+        for (size_t i = 0; i < m; ++i)
+            for (uint64_t r = qm[i]; r; r &= r - 1) list[fill[__builtin_ctzll(r)]++] = (uint32_t)i;
+    }
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    auto front = [&](uint32_t q, const std::vector<uint32_t> &h) {
+        return h[q] < start[q + 1] ? list[h[q]] : kNone;
+    };
+    auto ready = [&](uint32_t i, const std::vector<uint32_t> &h) {
+        for (uint64_t r = qm[i]; r; r &= r - 1)
+            if (front(__builtin_ctzll(r), h) != i) return false;
+        return true;
+    };
+    auto allowed = [&](uint32_t i, uint64_t tile) {
+        return is_diagonal(ops[i].kind) || ((tile >> ops[i].target) & 1ull);
+    };
+    std::vector<uint32_t> work;
+    uint32_t pass_limited = 0;  // limited ops in the pass being built
+    // run every gate that can run in `tile`; appends them to out when given
+    // (and then counts the limited ones into the pass)
+    auto closure = [&](std::vector<uint32_t> &h, uint64_t tile, std::vector<uint32_t> *out) {
+        size_t count = 0;
+        uint32_t nl = pass_limited;
+        work.clear();
+        for (uint32_t q = 0; q < n; ++q) work.push_back(q);
+        while (!work.empty()) {
+            const uint32_t q = work.back();
+            work.pop_back();
+            const uint32_t i = front(q, h);
+            if (i == kNone || !ready(i, h) || !allowed(i, tile)) continue;
+            const bool lim = limited && (*limited)[i];
+            if (lim && nl >= max_limited) continue;
+            nl += lim;
+            for (uint64_t r = qm[i]; r; r &= r - 1) {
+                const uint32_t p = __builtin_ctzll(r);
+                ++h[p];
+                work.push_back(p);
+            }
+            ++count;
+            if (out) out->push_back(i);
+        }
+        if (out) pass_limited = nl;
+        return count;
+    };
+    std::vector<uint32_t> head(start.begin(), start.end() - 1), trial;
+    size_t left = m;
+    order.clear();
+    ends.clear();
+    while (left) {
+        uint64_t tile = 7;
+        pass_limited = 0;
+        left -= closure(head, tile, &order);
+        for (uint32_t used = 0; used < free_targets && left; ++used) {
+            uint64_t cands = 0;
+            for (uint32_t q = 0; q < n; ++q) {
+                const uint32_t i = front(q, head);
+                if (i != kNone && !is_diagonal(ops[i].kind) && !((tile >> ops[i].target) & 1ull) && ready(i, head))
+                    cands |= 1ull << ops[i].target;
+            }
+            if (!cands) break;
+            uint32_t best = 0;
+            size_t best_count = 0;
+            for (uint64_t r = cands; r; r &= r - 1) {
+                const uint32_t t = __builtin_ctzll(r);
+                trial = head;
+                const size_t c = closure(trial, tile | (1ull << t), nullptr);
+                if (c > best_count) {
+                    best_count = c;
+                    best = t;
+                }
+            }
+            tile |= 1ull << best;
+            left -= closure(head, tile, &order);
+        }
+        ends.push_back((uint32_t)order.size());
+    }
 }
 
 }  // namespace

@@ -1,0 +1,72 @@
+// The tiled pass kernel k_tile, shared by the exact tiled runs (tiled.hip,
+// built with FP contraction off) and the contracted copy used by dependency-
+// ordered runs (tiled_fma.hip).  Include after tile_core.h.
+#pragma once
+
+namespace kbwh {
+namespace {
+
+using namespace tile;
+
+// One tile per workgroup: load the tile into registers, apply the gates,
+// store it back; overlap comes from the other resident workgroups.
+// PERM: Block swaps folded into the pass; the tile is stored through st, whose
+// qubits are those of ts in another order (position b of the tile goes to
+// qubit st.qbit[b]), so exchanged qubits cost no extra traffic.
+template <bool PERM>
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_tile(
+    dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles,
+    TileSpec st) {
+    __shared__ dv2 lds[TT * kChunk];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    uint64_t tmask = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b)
+        if ((tid >> b) & 1u) tmask |= ts.qbit[b];
+    const uint64_t base = tile_base(tile, ts);
+    const uint64_t dt = base | tmask;
+    dv2 v[TR];
+#pragma unroll
+    for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
+    // ng >= 1 (the host launches only non-empty passes); a do-while leaves no
+    // path around the loop, so the loaded values need no copy kept for one.
+    // The next gate's record is loaded while the current gate is applied.
+    uint32_t i = 0;
+    TGate G = gates[0];
+    do {
+        const TGate Gn = gates[i + 1 < ng ? i + 1 : i];
+        if ((base & G.ocm) == G.ocm) {  // a control outside the tile is 0 here
+            if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
+                apply_diag(v, base, tid, G);
+            else
+                apply_pair(v, lds, tid, G);
+        }
+        G = Gn;
+    } while (++i < ng);
+    // recompute the store addresses instead of keeping eight of them live
+    // through the gate loop
+    if constexpr (PERM) {
+        // the tile goes to addresses that other waves of the workgroup
+        // loaded: every load must have returned before any store is issued
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        uint64_t sm = 0;
+#pragma unroll
+        for (int b = 0; b < TB; ++b)
+            if ((tid >> b) & 1u) sm |= st.qbit[b];
+        uint64_t ds = base | sm;
+        asm volatile("" : "+v"(ds));
+#pragma unroll
+        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, st));
+    } else {
+        uint64_t ds = dt;
+        asm volatile("" : "+v"(ds));
+#pragma unroll
+        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, ts));
+    }
+}
+
+}  // namespace
+}  // namespace kbwh

@@ -25,6 +25,7 @@
 
 #include "state.h"
 #include "tile_core.h"
+#include "tile_pass.h"
 
 #pragma clang fp contract(off)
 namespace kbwh {
@@ -32,67 +33,6 @@ namespace kbwh {
 namespace {
 
 using namespace tile;
-
-// One tile per workgroup: load the tile into registers, apply the gates,
-// store it back; overlap comes from the other resident workgroups.
-// PERM: Block swaps folded into the pass; the tile is stored through st, whose
-// qubits are those of ts in another order (position b of the tile goes to
-// qubit st.qbit[b]), so exchanged qubits cost no extra traffic.
-template <bool PERM>
-__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_tile(
-    dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles,
-    TileSpec st) {
-    __shared__ dv2 lds[TT * kChunk];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t tile = blockIdx.x;
-    if (tile >= ntiles) return;
-    uint64_t tmask = 0;
-#pragma unroll
-    for (int b = 0; b < TB; ++b)
-        if ((tid >> b) & 1u) tmask |= ts.qbit[b];
-    const uint64_t base = tile_base(tile, ts);
-    const uint64_t dt = base | tmask;
-    dv2 v[TR];
-#pragma unroll
-    for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
-    // ng >= 1 (the host launches only non-empty passes); a do-while leaves no
-    // path around the loop, so the loaded values need no copy kept for one.
-    // The next gate's record is loaded while the current gate is applied.
-    uint32_t i = 0;
-    TGate G = gates[0];
-    do {
-        const TGate Gn = gates[i + 1 < ng ? i + 1 : i];
-        if ((base & G.ocm) == G.ocm) {  // a control outside the tile is 0 here
-            if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
-                apply_diag(v, base, tid, G);
-            else
-                apply_pair(v, lds, tid, G);
-        }
-        G = Gn;
-    } while (++i < ng);
-    // recompute the store addresses instead of keeping eight of them live
-    // through the gate loop
-    if constexpr (PERM) {
-        // the tile goes to addresses that other waves of the workgroup
-        // loaded: every load must have returned before any store is issued
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        uint64_t sm = 0;
-#pragma unroll
-        for (int b = 0; b < TB; ++b)
-            if ((tid >> b) & 1u) sm |= st.qbit[b];
-        uint64_t ds = base | sm;
-        asm volatile("" : "+v"(ds));
-#pragma unroll
-        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, st));
-    } else {
-        uint64_t ds = dt;
-        asm volatile("" : "+v"(ds));
-#pragma unroll
-        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, ts));
-    }
-}
-
 
 // One Pauli string of a grouped exp_value sweep: P|j> = i^ny (-1)^popcount(j & z) |j ^ x>,
 // with x as a mask of tile positions (xt); coeff is the string's weight.
@@ -349,101 +289,6 @@ int32_t pauli_apply_tiled(kbwh_state *s, const double2 *phi, double2 *lam, const
     return hip_status(e, "Pauli sum sweeps");
 }
 
-namespace {
-
-// Dependency-ordered pass planning (KBWH_RUN_REORDER).  Gates that share no
-// qubit (target or control) commute, so any order that keeps, on every qubit,
-// the order of the gates touching it is the same unitary; only the rounding
-// of the amplitudes can change.  A gate is ready when it is the first pending
-// gate on each of its qubits.  A pass starts with tile qubits 0..2 and runs
-// every ready gate it can (diagonal, or target in the tile), again and again;
-// while target positions are left, the target of a ready gate that lets the
-// most pending gates run (the lowest such qubit on ties) joins the tile.
-// order: op indices in execution order; ends[p]: end of pass p in order.
-void schedule_passes(const std::vector<PhysOp> &ops, uint32_t n, uint32_t free_targets,
-                     std::vector<uint32_t> &order, std::vector<uint32_t> &ends) {
-    const size_t m = ops.size();
-    std::vector<uint64_t> qm(m);
-    std::vector<uint32_t> start(n + 1, 0);
-    for (size_t i = 0; i < m; ++i) {
-        qm[i] = ops[i].ctrl | (1ull << ops[i].target);
-        for (uint64_t r = qm[i]; r; r &= r - 1) ++start[__builtin_ctzll(r) + 1];
-    }
-    for (uint32_t q = 0; q < n; ++q) start[q + 1] += start[q];
-    std::vector<uint32_t> list(start[n]);
-    {
-        std::vector<uint32_t> fill(start.begin(), start.end() - 1);
-        for (size_t i = 0; i < m; ++i)
-            for (uint64_t r = qm[i]; r; r &= r - 1) list[fill[__builtin_ctzll(r)]++] = (uint32_t)i;
-    }
-    constexpr uint32_t kNone = 0xFFFFFFFFu;
-    auto front = [&](uint32_t q, const std::vector<uint32_t> &h) {
-        return h[q] < start[q + 1] ? list[h[q]] : kNone;
-    };
-    auto ready = [&](uint32_t i, const std::vector<uint32_t> &h) {
-        for (uint64_t r = qm[i]; r; r &= r - 1)
-            if (front(__builtin_ctzll(r), h) != i) return false;
-        return true;
-    };
-    auto allowed = [&](uint32_t i, uint64_t tile) {
-        return is_diagonal(ops[i].kind) || ((tile >> ops[i].target) & 1ull);
-    };
-    std::vector<uint32_t> work;
-    // run every gate that can run in `tile`; appends them to out when given
-    auto closure = [&](std::vector<uint32_t> &h, uint64_t tile, std::vector<uint32_t> *out) {
-        size_t count = 0;
-        work.clear();
-        for (uint32_t q = 0; q < n; ++q) work.push_back(q);
-        while (!work.empty()) {
-            const uint32_t q = work.back();
-            work.pop_back();
-            const uint32_t i = front(q, h);
-            if (i == kNone || !ready(i, h) || !allowed(i, tile)) continue;
-            for (uint64_t r = qm[i]; r; r &= r - 1) {
-                const uint32_t p = __builtin_ctzll(r);
-                ++h[p];
-                work.push_back(p);
-            }
-            ++count;
-            if (out) out->push_back(i);
-        }
-        return count;
-    };
-    std::vector<uint32_t> head(start.begin(), start.end() - 1), trial;
-    size_t left = m;
-    order.clear();
-    ends.clear();
-    while (left) {
-        uint64_t tile = 7;
-        left -= closure(head, tile, &order);
-        for (uint32_t used = 0; used < free_targets && left; ++used) {
-            uint64_t cands = 0;
-            for (uint32_t q = 0; q < n; ++q) {
-                const uint32_t i = front(q, head);
-                if (i != kNone && !is_diagonal(ops[i].kind) && !((tile >> ops[i].target) & 1ull) && ready(i, head))
-                    cands |= 1ull << ops[i].target;
-            }
-            if (!cands) break;
-            uint32_t best = 0;
-            size_t best_count = 0;
-            for (uint64_t r = cands; r; r &= r - 1) {
-                const uint32_t t = __builtin_ctzll(r);
-                trial = head;
-                const size_t c = closure(trial, tile | (1ull << t), nullptr);
-                if (c > best_count) {
-                    best_count = c;
-                    best = t;
-                }
-            }
-            tile |= 1ull << best;
-            left -= closure(head, tile, &order);
-        }
-        ends.push_back((uint32_t)order.size());
-    }
-}
-
-}  // namespace
-
 }  // namespace kbwh
 
 // The dependency-ordered pass plan of KBWH_RUN_REORDER on the host only.
@@ -515,6 +360,9 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         uint32_t first, count;
         bool perm;
     };
+    static_assert(sizeof(TileSpec) == kTileSpecBytes && sizeof(TGate) == kTGateBytes,
+                  "tile records shared with tiled_fma.hip");
+    const bool contract = reorder && !s->block;
     std::vector<Launch> launches;
     const uint32_t cap = std::min<uint32_t>(s->tile_targets, TL - 3);
     auto flush = [&]() -> int32_t {
@@ -574,11 +422,15 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         const uint32_t grid = ntiles;
         const TGate *dg = (const TGate *)s->tile_gates;
         for (const Launch &L : launches) {
-            if (L.perm)
+            if (L.perm) {
                 k_tile<true><<<grid, TT, 0, s->stream>>>((dv2 *)amp, L.spec, dg + L.first, L.count, ntiles, L.store);
-            else
+                e = hipGetLastError();
+            } else if (contract) {
+                e = launch_tile_contracted(amp, &L.spec, dg + L.first, L.count, ntiles, s->stream);
+            } else {
                 k_tile<false><<<grid, TT, 0, s->stream>>>((dv2 *)amp, L.spec, dg + L.first, L.count, ntiles, L.spec);
-            e = hipGetLastError();
+                e = hipGetLastError();
+            }
             if (e != hipSuccess) return hip_status(e, "k_tile launch");
         }
         // the gate buffer is reused by the next run: the upload above is

@@ -1,0 +1,28 @@
+// The tiled pass kernel built with FP contraction (KBWH_RUN_REORDER only).
+//
+// A dependency-ordered run already differs from the program-order result by
+// rounding, so its passes use a copy of k_tile in which a product feeding an
+// add is fused into one FMA: the two-term pair form and the complex products
+// of phase gates take 4 FP64 operations per amplitude instead of 6.  Exact
+// runs (KBWH_RUN_TILED, Block states) never use this copy.
+#define KBWH_TILE_CONTRACT 1
+#include <string.h>
+
+#include "state.h"
+#include "tile_core.h"
+#include "tile_pass.h"
+
+namespace kbwh {
+
+hipError_t launch_tile_contracted(double2 *amp, const void *spec, const void *gates, uint32_t num_gates,
+                                  uint32_t ntiles, hipStream_t stream) {
+    static_assert(sizeof(tile::TileSpec) == kTileSpecBytes && sizeof(tile::TGate) == kTGateBytes,
+                  "tile records shared with tiled.hip");
+    tile::TileSpec ts;
+    memcpy(&ts, spec, sizeof ts);
+    k_tile<false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, (const tile::TGate *)gates, num_gates,
+                                                    ntiles, ts);
+    return hipGetLastError();
+}
+
+}  // namespace kbwh

@@ -294,21 +294,23 @@ __device__ __forceinline__ void apply_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid,
 
 // Z, P and RZ: each amplitude is multiplied by the phase of its half
 // (dense_v2.rs:190-191, 215-216, 271-272); Z and P leave the |0> half alone.
-// NEG: the Z gate, whose product by -1 is the negation in value.
-template <bool NEG>
+// Z's product by -1 is the negation in value.
+enum { DK_Z = 0, DK_P = 1, DK_RZ = 2 };
+
+template <int DK>
 __device__ __forceinline__ dv2 diag_val(dv2 x, dv2 ph) {
-    if constexpr (NEG)
+    if constexpr (DK == DK_Z)
         return -x;
     else
         return cmul(x, ph);
 }
 
-template <bool NEG, bool CTRL>
+template <int DK, bool CTRL>
 __device__ __forceinline__ void diag_all(dv2 (&v)[TR], uint32_t tid, uint32_t cm, dv2 ph) {
     const bool tok = thread_ctrl_ok(tid, cm);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
-        const dv2 out = diag_val<NEG>(v[r], ph);
+        const dv2 out = diag_val<DK>(v[r], ph);
         if constexpr (CTRL) {
             v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
         } else {
@@ -317,50 +319,59 @@ __device__ __forceinline__ void diag_all(dv2 (&v)[TR], uint32_t tid, uint32_t cm
     }
 }
 
-// target on register bit rb: slot r is in the |1> half when bit rb of r is
-// set.  Branch-free (every slot computed, kept by a uniform select): a branch
-// per slot on a runtime rb was turned back into a switch over the positions,
-// with pointer phis into v[] that demoted the register array to scratch.
-template <bool NEG, bool CTRL>
-__device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm, uint32_t rb, bool rz,
-                                         dv2 ph0, dv2 ph1) {
+// Target on register bit rb: slot r is in the |1> half when bit rb of r is
+// set (a uniform test per slot, so the phase is picked by scalar selects).
+// RZ changes every slot; Z and P keep the |0> slots by a uniform select.  The
+// bit stays a runtime value: register-bit templates here (each variant
+// touching a different subset of slots) were merged by the optimizer into
+// pointer selects over v[] that demoted the register array to scratch.
+template <int DK, bool CTRL>
+__device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm, uint32_t rb, dv2 ph0,
+                                         dv2 ph1) {
     const bool tok = thread_ctrl_ok(tid, cm);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
         const bool one = ((uint32_t)r >> rb) & 1u;
-        const dv2 out = diag_val<NEG>(v[r], one ? ph1 : ph0);
-        bool apply = one || rz;
+        const dv2 out = diag_val<DK>(v[r], one ? ph1 : ph0);
+        bool apply = DK == DK_RZ || one;
         if constexpr (CTRL) apply = apply && slot_ctrl_ok(tok, r, cm);
-        v[r] = apply ? out : v[r];
+        if (DK == DK_RZ && !CTRL)
+            v[r] = out;
+        else
+            v[r] = apply ? out : v[r];
     }
 }
 
-template <bool NEG, bool CTRL>
+template <int DK, bool CTRL>
 __device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
     const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
     const uint32_t cm = G.cm, tp = G.tpos;
-    const bool rz = G.kind == KRZ;
-    // one range test and a select (no multi-way branch on tp; see apply_pair_t)
+    // one range test (no multi-way branch on tp; see apply_pair_t)
     if (tp - (uint32_t)TB < 8u) {
-        diag_reg<NEG, CTRL>(v, tid, cm, tp - (uint32_t)TB, rz, ph0, ph1);
+        diag_reg<DK, CTRL>(v, tid, cm, tp - (uint32_t)TB, ph0, ph1);
         return;
     }
     // the target bit is fixed per tile (outside Q) or per thread (lane/wave bit)
     const bool one = tp == kOutside ? (base & G.otb) != 0 : ((tid >> (tp & 31u)) & 1u) != 0;
-    if (!one && !rz) return;
-    diag_all<NEG, CTRL>(v, tid, cm, one ? ph1 : ph0);
+    if (DK != DK_RZ && !one) return;
+    diag_all<DK, CTRL>(v, tid, cm, one ? ph1 : ph0);
 }
 
 __device__ __forceinline__ void apply_diag(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
     if (G.kind == KZ) {
         if (G.cm == 0)
-            apply_diag_t<true, false>(v, base, tid, G);
+            apply_diag_t<DK_Z, false>(v, base, tid, G);
         else
-            apply_diag_t<true, true>(v, base, tid, G);
+            apply_diag_t<DK_Z, true>(v, base, tid, G);
+    } else if (G.kind == KP) {
+        if (G.cm == 0)
+            apply_diag_t<DK_P, false>(v, base, tid, G);
+        else
+            apply_diag_t<DK_P, true>(v, base, tid, G);
     } else if (G.cm == 0) {
-        apply_diag_t<false, false>(v, base, tid, G);
+        apply_diag_t<DK_RZ, false>(v, base, tid, G);
     } else {
-        apply_diag_t<false, true>(v, base, tid, G);
+        apply_diag_t<DK_RZ, true>(v, base, tid, G);
     }
 }
 

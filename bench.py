@@ -60,11 +60,12 @@ def parse():
 
 
 def pmc_run(n):
-    """A 1 GiB device copy (the counter scale), 4 uncontrolled pair passes and
-    the first 3 tiled passes of the brickwork circuit."""
+    """A 1 GiB device copy (the counter scale), 4 uncontrolled pair passes,
+    the first tiled passes of the brickwork circuit (k_tile) and the same
+    gates in dependency-ordered passes (k_tile_fma)."""
     import torch
 
-    from kbw_amd import DenseHip, RUN_TILED, circuits
+    from kbw_amd import DenseHip, RUN_REORDER, RUN_TILED, circuits
 
     a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
     a.fill_(1.0)
@@ -77,8 +78,11 @@ def pmc_run(n):
     ops, _ = circuits.brickwork(n, 20, seed=1)
     s.run(ops[:60], mode=RUN_TILED)
     s.synchronize()
+    tiled_passes = s.run_stats()[1]
+    s.run(ops[:120], mode=RUN_REORDER)
+    s.synchronize()
     print(json.dumps({"pmc_run": True, "copy_bytes": 1 << 30, "pass_bytes": 32 << n,
-                      "tiled_passes": s.run_stats()[1]}), flush=True)
+                      "tiled_passes": tiled_passes, "reorder_passes": s.run_stats()[1]}), flush=True)
 
 
 def block_leg(shots=1_000_000):

@@ -199,6 +199,44 @@ __device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, const TGate
     }
 }
 
+// Lane exchanges.  xor 1 and xor 2 are DPP quad permutes and xor 8 a DPP
+// rotate of each 16-lane row by 8 (VALU moves, no LDS traffic); xor 4, 16
+// and 32 go through ds_bpermute.  Lane layouts pinned on gfx950: quad_perm
+// 0xB1 gives lane i the value of lane i^1, 0x4E of lane i^2, row_ror:8 of
+// lane i^8.
+enum { LX_Q1 = 0, LX_Q2 = 1, LX_R8 = 2, LX_SHFL = 3 };
+
+template <int LX>
+__device__ __forceinline__ double lane_xor_d(double x, int lm) {
+    if constexpr (LX == LX_SHFL) {
+        return __shfl_xor(x, lm, 64);
+    } else {
+        constexpr int ctrl = LX == LX_Q1 ? 0xB1 : LX == LX_Q2 ? 0x4E : 0x128;
+        const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+        const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, ctrl, 0xF, 0xF, false);
+        const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), ctrl, 0xF, 0xF, false);
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    }
+}
+
+template <int PK, bool CTRL, int LX>
+__device__ __forceinline__ void lane_pair(dv2 (&v)[TR], uint32_t tid, uint32_t tp, const TGate &G) {
+    const uint32_t cm = G.cm;
+    const bool tok = thread_ctrl_ok(tid, cm);
+    const bool one = ((tid >> tp) & 1u) != 0;
+    const int lm = 1 << tp;
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const dv2 o = mk2(lane_xor_d<LX>(v[r].x, lm), lane_xor_d<LX>(v[r].y, lm));
+        const dv2 out = pair_val<PK>(v[r], o, one, G);
+        if constexpr (CTRL) {
+            v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
+        } else {
+            v[r] = out;
+        }
+    }
+}
+
 template <int PK, bool CTRL>
 __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, uint32_t tp,
                                             const TGate &G) {
@@ -206,16 +244,18 @@ __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid
     const bool tok = thread_ctrl_ok(tid, cm);
     const bool one = ((tid >> tp) & 1u) != 0;
     if (tp < 6) {
-        const int lm = 1 << tp;
-#pragma unroll
-        for (int r = 0; r < TR; ++r) {
-            const dv2 o = shfl_xor2(v[r], lm);
-            const dv2 out = pair_val<PK>(v[r], o, one, G);
-            if constexpr (CTRL) {
-                v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
-            } else {
-                v[r] = out;
-            }
+        // dispatched on the position's bits (no multi-way branch; see apply_pair_t)
+        if (tp & 4u) {
+            lane_pair<PK, CTRL, LX_SHFL>(v, tid, tp, G);
+        } else if (tp & 2u) {
+            if (tp & 1u)
+                lane_pair<PK, CTRL, LX_R8>(v, tid, tp, G);
+            else
+                lane_pair<PK, CTRL, LX_SHFL>(v, tid, tp, G);
+        } else if (tp & 1u) {
+            lane_pair<PK, CTRL, LX_Q2>(v, tid, tp, G);
+        } else {
+            lane_pair<PK, CTRL, LX_Q1>(v, tid, tp, G);
         }
         return;
     }

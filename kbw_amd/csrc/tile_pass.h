@@ -3,6 +3,11 @@
 // ordered runs (tiled_fma.hip).  Include after tile_core.h.
 #pragma once
 
+// the kernel's name: the contracted copy is named apart in kernel traces
+#ifndef KBWH_TILE_PASS_NAME
+#define KBWH_TILE_PASS_NAME k_tile
+#endif
+
 namespace kbwh {
 namespace {
 
@@ -14,7 +19,7 @@ using namespace tile;
 // qubits are those of ts in another order (position b of the tile goes to
 // qubit st.qbit[b]), so exchanged qubits cost no extra traffic.
 template <bool PERM>
-__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_tile(
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void KBWH_TILE_PASS_NAME(
     dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles,
     TileSpec st) {
     __shared__ dv2 lds[TT * kChunk];

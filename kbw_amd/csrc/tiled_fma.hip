@@ -6,6 +6,7 @@
 // of phase gates take 4 FP64 operations per amplitude instead of 6.  Exact
 // runs (KBWH_RUN_TILED, Block states) never use this copy.
 #define KBWH_TILE_CONTRACT 1
+#define KBWH_TILE_PASS_NAME k_tile_fma
 #include <string.h>
 
 #include "state.h"
@@ -20,7 +21,7 @@ hipError_t launch_tile_contracted(double2 *amp, const void *spec, const void *ga
                   "tile records shared with tiled.hip");
     tile::TileSpec ts;
     memcpy(&ts, spec, sizeof ts);
-    k_tile<false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, (const tile::TGate *)gates, num_gates,
+    k_tile_fma<false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, (const tile::TGate *)gates, num_gates,
                                                     ntiles, ts);
     return hipGetLastError();
 }

@@ -355,6 +355,7 @@ int32_t build_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kb
             cur = Pass();
         };
         std::vector<int32_t> pidx;
+        std::vector<PhysOp> phys;
         for (uint64_t i = K.first_op; i < K.first_op + K.num_ops; ++i) {
             const kbwh_op &op = ops[i];
             if (op.kind > KRZ || op.target >= n || (op.ctrl_mask >> n))
@@ -367,20 +368,25 @@ int32_t build_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kb
                 if (pi >= num_params) return set_error(KBWH_ERR_INVALID_ARGUMENT, "param index out of range");
                 gp = (int32_t)pi;
             }
-            const PhysOp po{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), op.angle};
-            if (!fits(cur, po)) {
-                // remember parameter indices of the pass being closed
-                close();
-            }
-            add_op(cur, po);
+            phys.push_back(PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), op.angle});
             pidx.push_back(gp);
         }
-        close();
+        // dependency-ordered passes (as KBWH_RUN_REORDER): batch energies
+        // agree with the reference to rounding, and the order only moves
+        // gates past gates on other qubits
+        std::vector<uint32_t> order, ends;
+        schedule_passes(phys, n_eff, TL - 3, order, ends);
+        size_t k0 = 0;
+        for (const uint32_t end : ends) {
+            for (size_t k = k0; k < end; ++k) add_op(cur, phys[order[k]]);
+            close();
+            k0 = end;
+        }
         const uint32_t gate_passes = (uint32_t)passes.size() - B.first_pass;
-        // attach parameter indices to this circuit's gate records, in order
+        // attach parameter indices to this circuit's gate records, in execution order
         if (gate_passes) {
             const size_t gi = passes[B.first_pass].first_gate;
-            for (size_t k = 0; k < pidx.size(); ++k) tmpl[gi + k].pad = (uint32_t)pidx[k];
+            for (size_t k = 0; k < order.size(); ++k) tmpl[gi + k].pad = (uint32_t)pidx[order[k]];
         } else {
             // no gates: one empty pass prepares and stores |0...0>
             uint32_t q_at[TL];

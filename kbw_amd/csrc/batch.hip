@@ -196,10 +196,16 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
                     double acc = 0.0;
                     for (uint32_t k = 0; k < P.num_str; ++k) {
                         const BString S = strings[P.first_str + k];
+                        // popcount(i & z) = popcount(dt & z) + popcount(r & zr), zr = z on the register bits
+                        uint32_t zr = 0;
+#pragma unroll
+                        for (int b = 0; b < TL - TB; ++b)
+                            if (P.ts.qbit[TB + b] & S.z) zr |= 1u << b;
+                        const bool pd = (__popcll(dt & S.z) & 1) != 0;
                         double sk = 0.0;
 #pragma unroll
                         for (int r = 0; r < TR; ++r) {
-                            const bool neg = __popcll(slot_offset(dt, r, P.ts) & S.z) & 1;
+                            const bool neg = pd != ((__builtin_popcount((uint32_t)r & zr) & 1) != 0);
                             double term;
                             bool flip = neg;
                             if (S.xt == 0) {

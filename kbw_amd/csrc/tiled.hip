@@ -80,10 +80,16 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
         }
         for (uint32_t k = 0; k < ns; ++k) {
             const SString S = ss[k];
+            // popcount(i & z) = popcount(dt & z) + popcount(r & zr), zr = z on the register bits
+            uint32_t zr = 0;
+#pragma unroll
+            for (int b = 0; b < TL - TB; ++b)
+                if (ts.qbit[TB + b] & S.z) zr |= 1u << b;
+            const bool pd = (__popcll(dt & S.z) & 1) != 0;
             double sk = 0.0;
 #pragma unroll
             for (int r = 0; r < TR; ++r) {
-                bool flip = __popcll(slot_offset(dt, r, ts) & S.z) & 1;
+                bool flip = pd != ((__builtin_popcount((uint32_t)r & zr) & 1) != 0);
                 double term;
                 if (S.xt == 0) {
                     term = v[r].x * v[r].x + v[r].y * v[r].y;

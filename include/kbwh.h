@@ -68,7 +68,8 @@ extern "C" {
 #define KBWH_RUN_FUSED 2u    /* gates fused into k-qubit blocks applied with FP64 MFMA; equal to rounding (~1e-16 per block), not bit for bit */
 #define KBWH_RUN_AUTO 3u     /* the configuration chosen by kbwh_autotune for this handle (KBWH_RUN_TILED until tuned) */
 #define KBWH_RUN_REORDER 4u  /* tiled passes over a dependency order of the gates (gates on disjoint qubits may trade places):
-                                the same unitary, amplitudes equal to rounding; fewer passes.  Block states run as KBWH_RUN_TILED */
+                                the same unitary, amplitudes equal to rounding; fewer passes.  A Block state keeps the reference's
+                                map and swap count (decided in program order); its swaps are ordered with the gates */
 
 /* One circuit operation (24 bytes).  `angle` is used unless `param` >= 0
  * and a parameter vector is supplied, in which case the angle is

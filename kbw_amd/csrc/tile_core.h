@@ -557,26 +557,25 @@ inline void add_op(Pass &p, const PhysOp &o) {
     p.ops.push_back(o);
 }
 
-// Dependency-ordered pass planning (KBWH_RUN_REORDER).  Gates that share no
-// qubit (target or control) commute, so any order that keeps, on every qubit,
-// the order of the gates touching it is the same unitary; only the rounding
-// of the amplitudes can change.  A gate is ready when it is the first pending
-// gate on each of its qubits.  A pass starts with tile qubits 0..2 and runs
-// every ready gate it can (diagonal, or target in the tile), again and again;
-// while target positions are left, the target of a ready gate that lets the
-// most pending gates run (the lowest such qubit on ties) joins the tile.
+// Dependency-ordered pass planning (KBWH_RUN_REORDER).  Ops that share no
+// qubit commute, so any order that keeps, on every qubit, the order of the ops
+// touching it is the same unitary; only the rounding of the amplitudes can
+// change.  Op i touches the qubits of qm[i] and needs the qubits of req[i] in
+// the tile (a non-diagonal gate its target, a Block swap both its qubits).
+// An op is ready when it is the first pending op on each of its qubits.  A
+// pass starts with tile qubits 0..2 and runs every ready op whose qubits are
+// in the tile, again and again; while target positions are left, the missing
+// qubits of the ready op that lets the most pending ops run (the lowest such
+// set on ties) join the tile.
 // order: op indices in execution order; ends[p]: end of pass p in order.
 // limited (optional): ops of which a pass may hold at most max_limited.
-inline void schedule_passes(const std::vector<PhysOp> &ops, uint32_t n, uint32_t free_targets,
-                            std::vector<uint32_t> &order, std::vector<uint32_t> &ends,
-                            const std::vector<uint8_t> *limited = nullptr, uint32_t max_limited = 0) {
-    const size_t m = ops.size();
-    std::vector<uint64_t> qm(m);
+inline void schedule_ops(const std::vector<uint64_t> &qm, const std::vector<uint64_t> &req, uint32_t n,
+                         uint32_t free_targets, std::vector<uint32_t> &order, std::vector<uint32_t> &ends,
+                         const std::vector<uint8_t> *limited = nullptr, uint32_t max_limited = 0) {
+    const size_t m = qm.size();
     std::vector<uint32_t> start(n + 1, 0);
-    for (size_t i = 0; i < m; ++i) {
-        qm[i] = ops[i].ctrl | (1ull << ops[i].target);
+    for (size_t i = 0; i < m; ++i)
         for (uint64_t r = qm[i]; r; r &= r - 1) ++start[__builtin_ctzll(r) + 1];
-    }
     for (uint32_t q = 0; q < n; ++q) start[q + 1] += start[q];
     std::vector<uint32_t> list(start[n]);
     {
@@ -593,12 +592,9 @@ inline void schedule_passes(const std::vector<PhysOp> &ops, uint32_t n, uint32_t
             if (front(__builtin_ctzll(r), h) != i) return false;
         return true;
     };
-    auto allowed = [&](uint32_t i, uint64_t tile) {
-        return is_diagonal(ops[i].kind) || ((tile >> ops[i].target) & 1ull);
-    };
     std::vector<uint32_t> work;
     uint32_t pass_limited = 0;  // limited ops in the pass being built
-    // run every gate that can run in `tile`; appends them to out when given
+    // run every op that can run in `tile`; appends them to out when given
     // (and then counts the limited ones into the pass)
     auto closure = [&](std::vector<uint32_t> &h, uint64_t tile, std::vector<uint32_t> *out) {
         size_t count = 0;
@@ -609,7 +605,7 @@ inline void schedule_passes(const std::vector<PhysOp> &ops, uint32_t n, uint32_t
             const uint32_t q = work.back();
             work.pop_back();
             const uint32_t i = front(q, h);
-            if (i == kNone || !ready(i, h) || !allowed(i, tile)) continue;
+            if (i == kNone || (req[i] & ~tile) || !ready(i, h)) continue;
             const bool lim = limited && (*limited)[i];
             if (lim && nl >= max_limited) continue;
             nl += lim;
@@ -625,37 +621,60 @@ inline void schedule_passes(const std::vector<PhysOp> &ops, uint32_t n, uint32_t
         return count;
     };
     std::vector<uint32_t> head(start.begin(), start.end() - 1), trial;
+    std::vector<uint64_t> cands;
     size_t left = m;
     order.clear();
     ends.clear();
     while (left) {
         uint64_t tile = 7;
+        uint32_t used = 0;
         pass_limited = 0;
+        const size_t before = order.size();
         left -= closure(head, tile, &order);
-        for (uint32_t used = 0; used < free_targets && left; ++used) {
-            uint64_t cands = 0;
+        while (used < free_targets && left) {
+            // the missing qubits of each ready op that still fit
+            cands.clear();
             for (uint32_t q = 0; q < n; ++q) {
                 const uint32_t i = front(q, head);
-                if (i != kNone && !is_diagonal(ops[i].kind) && !((tile >> ops[i].target) & 1ull) && ready(i, head))
-                    cands |= 1ull << ops[i].target;
+                if (i == kNone) continue;
+                const uint64_t miss = req[i] & ~tile;
+                if (!miss || used + (uint32_t)__builtin_popcountll(miss) > free_targets || !ready(i, head)) continue;
+                if (std::find(cands.begin(), cands.end(), miss) == cands.end()) cands.push_back(miss);
             }
-            if (!cands) break;
-            uint32_t best = 0;
+            if (cands.empty()) break;
+            std::sort(cands.begin(), cands.end());
+            uint64_t best = cands[0];
             size_t best_count = 0;
-            for (uint64_t r = cands; r; r &= r - 1) {
-                const uint32_t t = __builtin_ctzll(r);
+            for (const uint64_t c : cands) {
                 trial = head;
-                const size_t c = closure(trial, tile | (1ull << t), nullptr);
-                if (c > best_count) {
-                    best_count = c;
-                    best = t;
+                const size_t k = closure(trial, tile | c, nullptr);
+                if (k > best_count) {
+                    best_count = k;
+                    best = c;
                 }
             }
-            tile |= 1ull << best;
+            tile |= best;
+            used += (uint32_t)__builtin_popcountll(best);
             left -= closure(head, tile, &order);
         }
+        // an op needing more tile positions than there are stops the plan
+        // short (order then holds fewer than m ops; callers check)
+        if (order.size() == before) break;
         ends.push_back((uint32_t)order.size());
     }
+}
+
+// schedule_ops for gates: a gate touches its target and controls and needs
+// its target in the tile unless it is diagonal.
+inline void schedule_passes(const std::vector<PhysOp> &ops, uint32_t n, uint32_t free_targets,
+                            std::vector<uint32_t> &order, std::vector<uint32_t> &ends,
+                            const std::vector<uint8_t> *limited = nullptr, uint32_t max_limited = 0) {
+    std::vector<uint64_t> qm(ops.size()), req(ops.size());
+    for (size_t i = 0; i < ops.size(); ++i) {
+        qm[i] = ops[i].ctrl | (1ull << ops[i].target);
+        req[i] = is_diagonal(ops[i].kind) ? 0ull : (1ull << ops[i].target);
+    }
+    schedule_ops(qm, req, n, free_targets, order, ends, limited, max_limited);
 }
 
 }  // namespace

@@ -456,6 +456,65 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         if (q >= 3 && std::find(cur.targets.begin(), cur.targets.end(), q) == cur.targets.end())
             cur.targets.push_back(q);
     };
+    if (reorder && s->block && cap >= 2) {
+        // A Block state: the LRU map and its swaps follow program order (the
+        // map is replayed here first, exactly as gate by gate), then swaps and
+        // gates are scheduled as ops on physical qubits.  A swap needs both of
+        // its qubits in the tile and is folded into its pass as above.  The
+        // ops touching each physical qubit keep their order, so the relabelling
+        // of every qubit is the same as in program order.
+        struct Ev {
+            PhysOp op;
+            uint32_t sl, sg;  // sl != kNoSwap: an exchange of physical qubits sl and sg
+        };
+        std::vector<Ev> ev;
+        ev.reserve(num_ops + num_ops / 4);
+        for (uint64_t i = 0; i < num_ops; ++i) {
+            const kbwh_op &op = ops[i];
+            const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
+            uint32_t t = op.target, sl = kNoSwap, sg = kNoSwap;
+            map_qubit(s, op.target, &t, &sl, &sg);
+            if (sl != kNoSwap) ev.push_back(Ev{PhysOp{KX, 0, 0, 0.0}, sl, sg});
+            const uint64_t m = physical_mask(s, op.ctrl_mask) & ~(1ull << t);
+            ev.push_back(Ev{PhysOp{(int)op.kind, t, m, angle}, kNoSwap, kNoSwap});
+        }
+        std::vector<uint64_t> qm(ev.size()), req(ev.size());
+        for (size_t i = 0; i < ev.size(); ++i) {
+            const Ev &E = ev[i];
+            if (E.sl != kNoSwap) {
+                qm[i] = req[i] = (1ull << E.sl) | (1ull << E.sg);
+            } else {
+                qm[i] = E.op.ctrl | (1ull << E.op.target);
+                req[i] = is_diagonal(E.op.kind) ? 0ull : (1ull << E.op.target);
+            }
+        }
+        std::vector<uint32_t> order, ends;
+        schedule_ops(qm, req, n, cap, order, ends);
+        if (order.size() != ev.size()) return set_error(KBWH_ERR_DEVICE, "Block schedule incomplete");
+        size_t k = 0;
+        for (const uint32_t end : ends) {
+            for (; k < end; ++k) {
+                const Ev &E = ev[order[k]];
+                if (E.sl != kNoSwap) {
+                    require(ld[E.sl]);
+                    require(ld[E.sg]);
+                    std::swap(ld[E.sl], ld[E.sg]);
+                    ph[ld[E.sl]] = E.sl;
+                    ph[ld[E.sg]] = E.sg;
+                    swapped = true;
+                    continue;
+                }
+                uint64_t lm = 0;
+                for (uint64_t r = E.op.ctrl; r; r &= r - 1) lm |= 1ull << ld[__builtin_ctzll(r)];
+                add_op(cur, PhysOp{E.op.kind, ld[E.op.target], lm, E.op.angle});
+            }
+            const int32_t rc = flush();
+            if (rc) return rc;
+        }
+        const int32_t rc = launch_all();
+        s->last_gates = num_ops;
+        return rc;
+    }
     if (reorder && !s->block) {
         // dependency-ordered passes (a Block state keeps program order: its
         // swaps follow the order in which targets are met)

@@ -1,7 +1,10 @@
-// Tile machinery shared by the tiled pass (tiled.hip) and the batched
-// persistent kernel (batch.hip): the tile layout, the gate record, the exact
-// per-amplitude gate application in registers, and the host-side planner
-// helpers.  Included by those two translation units only.
+// Tile machinery shared by the tiled passes (tiled.hip, tiled_fma.hip), the
+// batched persistent kernel (batch.hip), the gate queue (live.hip), the tiled
+// adjoint sweep (adjoint_tiled.hip) and the fused-block planner (fused.hip):
+// the tile layout, the gate record, the exact per-amplitude gate application
+// in registers, and the host-side planners (program order and dependency
+// order).  Each unit gets its own copy (internal linkage) and may pick its own
+// tile shape.
 #pragma once
 
 #include <stdint.h>

@@ -1,4 +1,5 @@
-// Multi-gate tiled passes (KBWH_RUN_TILED).
+// Multi-gate tiled passes (KBWH_RUN_TILED, KBWH_RUN_REORDER), Block swaps
+// folded into passes, and the grouped Pauli-string sweeps.
 //
 // A pass applies a run of consecutive gates to the whole state in one HBM
 // read and one HBM write.  The state is cut into tiles of 2^TL amplitudes
@@ -7,15 +8,17 @@
 // number.  Q always holds qubits 0..2, so every 128-byte line of the state
 // belongs to one tile and is read whole.  A workgroup of 256 threads holds
 // one tile in registers, 2^(TL-8) amplitudes per thread:
-//   tile position 0..5    -> lane bits      (exchange with cross-lane shuffles)
+//   tile position 0..5    -> lane bits      (DPP moves or cross-lane shuffles)
 //   tile position 6..7    -> wave bits      (exchange through LDS)
 //   tile position 8..TL-1 -> register index (no exchange)
 // A non-diagonal gate (X, Y, H, RX, RY) needs its target in Q; diagonal gates
 // (Z, P, RZ) and every control may sit anywhere (bits outside Q are constant
-// over a tile).  Gates are applied in program order and every output equals
-// the reference's per-pair closure (dense_v2.rs:166-277) in value, so a tiled
-// run produces the same amplitudes as one pass per gate; only the number of
-// HBM passes changes.
+// over a tile).  KBWH_RUN_TILED applies gates in program order and every
+// output equals the reference's per-pair closure (dense_v2.rs:166-277) in
+// value, so a tiled run produces the same amplitudes as one pass per gate;
+// only the number of HBM passes changes.  KBWH_RUN_REORDER plans the passes
+// in a dependency order of the gates (schedule_ops) and runs them in the
+// contracted kernel copy (tiled_fma.hip): the same unitary, equal to rounding.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>

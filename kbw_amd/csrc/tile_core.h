@@ -137,12 +137,12 @@ __device__ __forceinline__ dv2 pair_out(dv2 m, dv2 o, const PairCoef &k) {
 //     H:  |0>: (m + o) k,     |1>: (o - m) k         (k = 1/sqrt2)
 //     RX: (c m.re + s o.im, c m.im - s o.re)
 //     RY: c m + sigma s o
-// RX and RY share one path (PK_TWO): out = A m + B q with q = o, or
-// q = (o.im, -o.re) for RX (B = Cr), and no product by D = 1.  Controlled
-// gates other than X use the generic form (PK_GEN).  Every path is chosen by
-// uniform two-way tests on the gate record, never by a multi-way branch (see
-// apply_pair_t).
-enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2, PK_Y = 3, PK_H = 4 };
+// RY (PK_TWO) is out = A m + B o and RX (PK_RX) out = A m + Cr q with
+// q = (o.im, -o.re), with no product by D = 1; a path each, so no amplitude
+// picks its partner's form at run time.  Controlled gates other than X use
+// the generic form (PK_GEN).  Every path is chosen by uniform two-way tests on
+// the gate record, never by a multi-way branch (see apply_pair_t).
+enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2, PK_Y = 3, PK_H = 4, PK_RX = 5 };
 
 // Control test for the amplitude in register slot r of thread tid, whose tile
 // index is tid + TT r: the thread bits are tested once per gate (tok) and the
@@ -168,10 +168,10 @@ __device__ __forceinline__ dv2 pair_val(dv2 m, dv2 o, bool one, const TGate &G) 
         return mk2(sx * G.d, sy * G.d);
     } else if constexpr (PK == PK_TWO) {
         const PairCoef k = pair_coef(G, one ? 1.0 : -1.0);
-        const bool rot = G.kind == KRX;
-        const double b = rot ? k.Cr : k.B;
-        const double qx = rot ? o.y : o.x, qy = rot ? -o.x : o.y;
-        return mk2(k.A * m.x + b * qx, k.A * m.y + b * qy);
+        return mk2(k.A * m.x + k.B * o.x, k.A * m.y + k.B * o.y);
+    } else if constexpr (PK == PK_RX) {
+        const PairCoef k = pair_coef(G, one ? 1.0 : -1.0);
+        return mk2(k.A * m.x + k.Cr * o.y, k.A * m.y + k.Cr * -o.x);
     } else {
         return pair_out(m, o, pair_coef(G, one ? 1.0 : -1.0));
     }
@@ -331,6 +331,8 @@ __device__ __forceinline__ void apply_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid,
         apply_pair_t<PK_Y, false>(v, lds, tid, G);
     else if (G.kind == KH)
         apply_pair_t<PK_H, false>(v, lds, tid, G);
+    else if (G.kind == KRX)
+        apply_pair_t<PK_RX, false>(v, lds, tid, G);
     else
         apply_pair_t<PK_TWO, false>(v, lds, tid, G);
 }

@@ -106,13 +106,14 @@ def block_leg(shots=1_000_000):
     for mode, name in ((RUN_TILED, "tiled"), (RUN_REORDER, "reorder")):
         s.clear()
         s.synchronize()
+        swaps0 = s.swap_count()
         t0 = _t.perf_counter()
         s.run(ops, mode=mode)
         s.synchronize()
         t_run = _t.perf_counter() - t0
         gates, passes = s.run_stats()
         z = [s.exp_value(Hamiltonian([p], [1.0])) for p in probes]
-        runs[name] = {"run_s": t_run, "passes": int(passes), "swaps": s.swap_count(),
+        runs[name] = {"run_s": t_run, "passes": int(passes), "swaps": s.swap_count() - swaps0,
                       "gates_per_s": gates / t_run, "z": z}
     dz = float(np.max(np.abs(np.array(runs["reorder"]["z"]) - np.array(runs["tiled"]["z"]))))
     u = np.random.default_rng(9).random(shots)

@@ -238,6 +238,7 @@ int32_t kbwh_read_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, doub
 int32_t kbwh_write_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, const double *in);
 /* logical_to_physical[q] = physical bit of logical qubit q (identity unless Block). */
 int32_t kbwh_qubit_map(const kbwh_state *s, uint32_t *logical_to_physical);
+/* Block swaps made since the handle was created (kbwh_clear keeps the count). */
 int32_t kbwh_swap_count(const kbwh_state *s, uint64_t *swaps);
 /* Gates and HBM passes (kernel launches over the state) of the last kbwh_run. */
 int32_t kbwh_run_stats(const kbwh_state *s, uint64_t *gates, uint64_t *passes);

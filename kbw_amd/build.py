@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libkbwh.so")
 BUILD_DIR = os.path.join(ROOT, "_build", "kbwh")
 
-HIP_SOURCES = ["kernels.hip", "state.hip", "observables.hip", "tiled.hip", "adjoint.hip", "batch.hip", "fused.hip", "autotune.hip", "live.hip", "adjoint_tiled.hip", "tiled_fma.hip"]
+HIP_SOURCES = ["kernels.hip", "state.hip", "observables.hip", "tiled.hip", "adjoint.hip", "batch.hip", "fused.hip", "autotune.hip", "live.hip", "adjoint_tiled.hip", "tiled_fma.hip", "tiled_pass.hip"]
 CXX_SOURCES = ["coef.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -25,6 +25,11 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
 ]
+# The tiled pass kernels keep their uniform per-gate branches unstructurized:
+# each gate path then updates the register tile in place, instead of writing a
+# second register set that is copied back at the end of every gate.
+_UNIFORM_BRANCHES = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
+HIP_SOURCE_FLAGS = {"tiled_pass.hip": _UNIFORM_BRANCHES, "tiled_fma.hip": _UNIFORM_BRANCHES}
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
 
@@ -44,7 +49,7 @@ def build(jobs=8, verbose=False):
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD_DIR, src.replace(".hip", ".o"))
         objs.append(obj)
-        procs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        procs.append([HIPCC, *HIP_FLAGS, *HIP_SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
     for src in CXX_SOURCES:
         obj = os.path.join(BUILD_DIR, src.replace(".cpp", ".o"))
         objs.append(obj)

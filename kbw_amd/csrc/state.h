@@ -64,10 +64,15 @@ int32_t apply_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint6
 int32_t reduce_to_host(kbwh_state *s, double *out, uint32_t count);
 double uniform_scale(double total);
 int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params);
-// One pass of the contracted k_tile copy (tiled_fma.hip) over a flat state:
-// spec / gates are a tile::TileSpec and a device array of tile::TGate of the
-// 11-qubit tile (the records are plain data of these sizes).
+// One pass of the tiled kernel k_tile over a state: spec / gates are a
+// tile::TileSpec and a device array of tile::TGate of the 11-qubit tile (the
+// records are plain data of these sizes).  launch_tile_exact (tiled_pass.hip)
+// stores the tile through the TileSpec `store` when it is not null (Block
+// swaps folded into the pass); launch_tile_contracted (tiled_fma.hip) is the
+// FP-contracted copy for dependency-ordered runs of flat states.
 constexpr size_t kTileSpecBytes = 176, kTGateBytes = 144;
+hipError_t launch_tile_exact(double2 *amp, const void *spec, const void *store, const void *gates,
+                             uint32_t num_gates, uint32_t ntiles, hipStream_t stream);
 hipError_t launch_tile_contracted(double2 *amp, const void *spec, const void *gates, uint32_t num_gates,
                                   uint32_t ntiles, hipStream_t stream);
 // reorder: dependency-ordered passes (KBWH_RUN_REORDER; flat states only)

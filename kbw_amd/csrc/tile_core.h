@@ -1,5 +1,5 @@
-// Tile machinery shared by the tiled passes (tiled.hip, tiled_fma.hip), the
-// batched persistent kernel (batch.hip), the gate queue (live.hip), the tiled
+// Tile machinery shared by the tiled passes (planned in tiled.hip, built in
+// tiled_pass.hip and tiled_fma.hip), the batched persistent kernel (batch.hip), the gate queue (live.hip), the tiled
 // adjoint sweep (adjoint_tiled.hip) and the fused-block planner (fused.hip):
 // the tile layout, the gate record, the exact per-amplitude gate application
 // in registers, and the host-side planners (program order and dependency

@@ -1,6 +1,12 @@
-// The tiled pass kernel k_tile, shared by the exact tiled runs (tiled.hip,
-// built with FP contraction off) and the contracted copy used by dependency-
-// ordered runs (tiled_fma.hip).  Include after tile_core.h.
+// The tiled pass kernel k_tile, shared by the exact tiled runs
+// (tiled_pass.hip, built with FP contraction off) and the contracted copy used
+// by dependency-ordered runs (tiled_fma.hip).  Include after tile_core.h.
+//
+// Both units are compiled with uniform branches left unstructurized
+// (-structurizecfg-skip-uniform-regions): the per-gate dispatch below is a
+// chain of uniform tests, and structurizing it into guarded blocks kept the
+// loaded register tile live across every gate path, so each path wrote a
+// second register set that was copied back after every gate.
 #pragma once
 
 // the kernel's name: the contracted copy is named apart in kernel traces

@@ -28,7 +28,6 @@
 
 #include "state.h"
 #include "tile_core.h"
-#include "tile_pass.h"
 
 #pragma clang fp contract(off)
 namespace kbwh {
@@ -428,18 +427,13 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         if (e != hipSuccess) return hip_status(e, "tile gate upload");
         s->last_passes += launches.size();
         const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
-        const uint32_t grid = ntiles;
         const TGate *dg = (const TGate *)s->tile_gates;
         for (const Launch &L : launches) {
-            if (L.perm) {
-                k_tile<true><<<grid, TT, 0, s->stream>>>((dv2 *)amp, L.spec, dg + L.first, L.count, ntiles, L.store);
-                e = hipGetLastError();
-            } else if (contract) {
+            if (contract && !L.perm)
                 e = launch_tile_contracted(amp, &L.spec, dg + L.first, L.count, ntiles, s->stream);
-            } else {
-                k_tile<false><<<grid, TT, 0, s->stream>>>((dv2 *)amp, L.spec, dg + L.first, L.count, ntiles, L.spec);
-                e = hipGetLastError();
-            }
+            else
+                e = launch_tile_exact(amp, &L.spec, L.perm ? &L.store : nullptr, dg + L.first, L.count, ntiles,
+                                      s->stream);
             if (e != hipSuccess) return hip_status(e, "k_tile launch");
         }
         // the gate buffer is reused by the next run: the upload above is

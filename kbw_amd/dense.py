@@ -269,6 +269,7 @@ class DenseHip:
         return phys[src]
 
     def swap_count(self):
+        """Block swaps made since the handle was created (clear() keeps the count)."""
         out = C.c_uint64()
         check(self._lib.kbwh_swap_count(self._h, C.byref(out)))
         return out.value

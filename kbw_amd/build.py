@@ -25,11 +25,23 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
 ]
-# The tiled pass kernels keep their uniform per-gate branches unstructurized:
-# each gate path then updates the register tile in place, instead of writing a
-# second register set that is copied back at the end of every gate.
+# Units holding register-tile kernels (tile_core.h) keep the optimizer from
+# sinking code shared by the per-gate variants: merged variants index the
+# register tile through a select of addresses, which puts it in scratch memory.
+# The tiled pass kernels also keep their uniform per-gate branches
+# unstructurized: each gate path then updates the register tile in place,
+# instead of writing a second register set that is copied back after every
+# gate.
+_NO_SINK = ["-mllvm", "-simplifycfg-sink-common=false"]
 _UNIFORM_BRANCHES = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
-HIP_SOURCE_FLAGS = {"tiled_pass.hip": _UNIFORM_BRANCHES, "tiled_fma.hip": _UNIFORM_BRANCHES}
+HIP_SOURCE_FLAGS = {
+    "tiled_pass.hip": _NO_SINK + _UNIFORM_BRANCHES,
+    "tiled_fma.hip": _NO_SINK + _UNIFORM_BRANCHES,
+    "batch.hip": _NO_SINK,
+    "live.hip": _NO_SINK,
+    "adjoint_tiled.hip": _NO_SINK,
+    "fused.hip": _NO_SINK,
+}
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
 

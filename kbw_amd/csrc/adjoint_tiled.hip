@@ -32,6 +32,7 @@
 #undef KBWH_TILE_TBITS
 #define KBWH_TILE_BITS 12
 #define KBWH_TILE_TBITS 8
+#define KBWH_TILE_GATE_RBITS 3
 #include "tile_core.h"
 
 #pragma clang fp contract(off)

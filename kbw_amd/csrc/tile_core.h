@@ -48,6 +48,13 @@ constexpr int TB = KBWH_TILE_TBITS;  // thread bits: 6 lane bits, then TB - 6 wa
 constexpr int TT = 1 << TB;          // threads per tile
 constexpr int TR = 1 << (TL - TB);   // amplitudes per thread (register bits = TL - TB)
 static_assert(TB >= 6 && TL - TB >= 1 && TL - TB <= 4, "tile geometry");
+// register bits that gate targets and controls may occupy (a unit that keeps
+// the top register bit for its own use lowers this)
+#ifndef KBWH_TILE_GATE_RBITS
+#define KBWH_TILE_GATE_RBITS (KBWH_TILE_BITS - KBWH_TILE_TBITS)
+#endif
+constexpr int TGR = KBWH_TILE_GATE_RBITS;
+static_assert(TGR >= 1 && TGR <= TL - TB, "gate register bits");
 constexpr uint32_t kOutside = 255;
 #ifndef KBWH_TILE_CHUNK
 #define KBWH_TILE_CHUNK 4
@@ -182,6 +189,10 @@ __device__ __forceinline__ dv2 pair_val(dv2 m, dv2 o, bool one, const TGate &G) 
 // matches cm change.
 template <int RB, int PK, bool CTRL>
 __device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, const TGate &G) {
+    // An exchange has no instructions of its own: without this marker the
+    // optimizer folds the register-bit variants of X into selects between
+    // every candidate partner.
+    if constexpr (PK == PK_X) asm volatile("");
     const uint32_t cm = G.cm;
     const bool tok = thread_ctrl_ok(tid, cm);
 #pragma unroll
@@ -358,85 +369,120 @@ __device__ __forceinline__ void apply_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid,
 
 // Z, P and RZ: each amplitude is multiplied by the phase of its half
 // (dense_v2.rs:190-191, 215-216, 271-272); Z and P leave the |0> half alone.
-// Z's product by -1 is the negation in value.
+// Z's product by -1 is the negation in value, a flip of the two sign bits.
+//
+// Z and P change the amplitudes whose target and control bits are all 1.
+// Those bits split into thread bits (a sign mask or a phase per thread),
+// register bits (a mask MH of register slots, dispatched once per gate to code
+// that touches exactly the slots r with r & MH == MH) and bits outside the
+// tile (fixed per tile: the gate is skipped where one is 0).  A thread that is
+// not selected multiplies by exactly 1 + 0i, which returns the value (a zero
+// may change sign).  RZ changes every slot whose controls hold: the phase
+// comes per thread (target on a thread bit) or per slot (target on a register
+// bit, or outside the tile).  No per-amplitude branch, no per-slot branch.
 enum { DK_Z = 0, DK_P = 1, DK_RZ = 2 };
 
-template <int DK>
-__device__ __forceinline__ dv2 diag_val(dv2 x, dv2 ph) {
-    if constexpr (DK == DK_Z)
-        return -x;
-    else
-        return cmul(x, ph);
+template <int N>
+struct IC {
+    static constexpr int value = N;
+};
+
+// f(IC<m>()) for the runtime mask m of NB register bits, by tests of its bits
+// (no multi-way branch; see apply_pair_t)
+template <int NB, int ACC, class F>
+__device__ __forceinline__ void with_slot_mask(uint32_t m, F &f) {
+    if constexpr (NB == 0) {
+        f(IC<ACC>());
+    } else {
+        if (m & (1u << (NB - 1)))
+            with_slot_mask<NB - 1, ACC | (1 << (NB - 1))>(m, f);
+        else
+            with_slot_mask<NB - 1, ACC>(m, f);
+    }
 }
 
-template <int DK, bool CTRL>
-__device__ __forceinline__ void diag_all(dv2 (&v)[TR], uint32_t tid, uint32_t cm, dv2 ph) {
-    const bool tok = thread_ctrl_ok(tid, cm);
+// m = 0 or the sign bit: flips the sign of both parts when set
+__device__ __forceinline__ dv2 flip_sign(dv2 x, uint32_t m) {
+    const unsigned long long h = (unsigned long long)m << 32;
+    return mk2(__longlong_as_double(__double_as_longlong(x.x) ^ (long long)h),
+               __longlong_as_double(__double_as_longlong(x.y) ^ (long long)h));
+}
+
+template <int DK>
+__device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
+    const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
+    const uint32_t cm = G.cm, tp = G.tpos;
+    const uint32_t rb = tp - (uint32_t)TB;
+    const bool reg_t = rb < 8u;         // target on a register bit
+    const bool out_t = tp == kOutside;  // target outside the tile: its bit is fixed per tile
+    const bool thr_t = !reg_t && !out_t;
+    const uint32_t clo = cm & (uint32_t)(TT - 1), chi = cm >> TB;
+    if constexpr (DK == DK_Z || DK == DK_P) {
+        if (out_t && (base & G.otb) == 0) return;
+        const uint32_t mlo = clo | (thr_t ? 1u << tp : 0u);
+        const uint32_t mhi = chi | (reg_t ? 1u << rb : 0u);
+        const bool sel = (tid & mlo) == mlo;
+        if constexpr (DK == DK_Z) {
+            const uint32_t m = sel ? 0x80000000u : 0u;
+            auto body = [&](auto mh) {
+                constexpr int MH = decltype(mh)::value;
 #pragma unroll
-    for (int r = 0; r < TR; ++r) {
-        const dv2 out = diag_val<DK>(v[r], ph);
-        if constexpr (CTRL) {
-            v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
+                for (int r = 0; r < TR; ++r)
+                    if ((r & MH) == MH) v[r] = flip_sign(v[r], m);
+            };
+            with_slot_mask<TGR, 0>(mhi, body);
         } else {
-            v[r] = out;
+            const dv2 f = sel ? ph1 : mk2(1.0, 0.0);
+            auto body = [&](auto mh) {
+                constexpr int MH = decltype(mh)::value;
+#pragma unroll
+                for (int r = 0; r < TR; ++r)
+                    if ((r & MH) == MH) v[r] = cmul(v[r], f);
+            };
+            with_slot_mask<TGR, 0>(mhi, body);
+        }
+    } else {
+        const bool ok = (tid & clo) == clo;
+        if (!reg_t) {
+            // one phase per thread (target on a thread bit) or per tile (outside)
+            const bool one = thr_t ? ((tid >> (tp & 31u)) & 1u) != 0 : (base & G.otb) != 0;
+            const dv2 f = ok ? (one ? ph1 : ph0) : mk2(1.0, 0.0);
+            auto body = [&](auto mh) {
+                constexpr int MH = decltype(mh)::value;
+#pragma unroll
+                for (int r = 0; r < TR; ++r)
+                    if ((r & MH) == MH) v[r] = cmul(v[r], f);
+            };
+            with_slot_mask<TGR, 0>(chi, body);
+        } else if (cm == 0) {
+            // target on register bit RB: every slot, each with a fixed phase
+            auto body = [&](auto tb) {
+                constexpr int TM = decltype(tb)::value;
+                if constexpr (TM != 0 && (TM & (TM - 1)) == 0) {
+#pragma unroll
+                    for (int r = 0; r < TR; ++r) v[r] = cmul(v[r], (r & TM) ? ph1 : ph0);
+                }
+            };
+            with_slot_mask<TGR, 0>(1u << rb, body);
+        } else {
+            // controlled, target on a register bit: the phase of slot r is
+            // picked by uniform selects; slots and threads left out multiply by 1
+#pragma unroll
+            for (int r = 0; r < TR; ++r) {
+                const dv2 ph = ((uint32_t)r & chi) == chi ? (((r >> rb) & 1u) ? ph1 : ph0) : mk2(1.0, 0.0);
+                v[r] = cmul(v[r], ok ? ph : mk2(1.0, 0.0));
+            }
         }
     }
 }
 
-// Target on register bit rb: slot r is in the |1> half when bit rb of r is
-// set (a uniform test per slot, so the phase is picked by scalar selects).
-// RZ changes every slot; Z and P keep the |0> slots by a uniform select.  The
-// bit stays a runtime value: register-bit templates here (each variant
-// touching a different subset of slots) were merged by the optimizer into
-// pointer selects over v[] that demoted the register array to scratch.
-template <int DK, bool CTRL>
-__device__ __forceinline__ void diag_reg(dv2 (&v)[TR], uint32_t tid, uint32_t cm, uint32_t rb, dv2 ph0,
-                                         dv2 ph1) {
-    const bool tok = thread_ctrl_ok(tid, cm);
-#pragma unroll
-    for (int r = 0; r < TR; ++r) {
-        const bool one = ((uint32_t)r >> rb) & 1u;
-        const dv2 out = diag_val<DK>(v[r], one ? ph1 : ph0);
-        bool apply = DK == DK_RZ || one;
-        if constexpr (CTRL) apply = apply && slot_ctrl_ok(tok, r, cm);
-        if (DK == DK_RZ && !CTRL)
-            v[r] = out;
-        else
-            v[r] = apply ? out : v[r];
-    }
-}
-
-template <int DK, bool CTRL>
-__device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
-    const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
-    const uint32_t cm = G.cm, tp = G.tpos;
-    // one range test (no multi-way branch on tp; see apply_pair_t)
-    if (tp - (uint32_t)TB < 8u) {
-        diag_reg<DK, CTRL>(v, tid, cm, tp - (uint32_t)TB, ph0, ph1);
-        return;
-    }
-    // the target bit is fixed per tile (outside Q) or per thread (lane/wave bit)
-    const bool one = tp == kOutside ? (base & G.otb) != 0 : ((tid >> (tp & 31u)) & 1u) != 0;
-    if (DK != DK_RZ && !one) return;
-    diag_all<DK, CTRL>(v, tid, cm, one ? ph1 : ph0);
-}
-
 __device__ __forceinline__ void apply_diag(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
-    if (G.kind == KZ) {
-        if (G.cm == 0)
-            apply_diag_t<DK_Z, false>(v, base, tid, G);
-        else
-            apply_diag_t<DK_Z, true>(v, base, tid, G);
-    } else if (G.kind == KP) {
-        if (G.cm == 0)
-            apply_diag_t<DK_P, false>(v, base, tid, G);
-        else
-            apply_diag_t<DK_P, true>(v, base, tid, G);
-    } else if (G.cm == 0) {
-        apply_diag_t<DK_RZ, false>(v, base, tid, G);
-    } else {
-        apply_diag_t<DK_RZ, true>(v, base, tid, G);
-    }
+    if (G.kind == KZ)
+        apply_diag_t<DK_Z>(v, base, tid, G);
+    else if (G.kind == KP)
+        apply_diag_t<DK_P>(v, base, tid, G);
+    else
+        apply_diag_t<DK_RZ>(v, base, tid, G);
 }
 
 // The tile number spread over the bits outside Q (a 0 inserted at each position of Q).

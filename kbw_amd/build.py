@@ -37,9 +37,9 @@ _UNIFORM_BRANCHES = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
 HIP_SOURCE_FLAGS = {
     "tiled_pass.hip": _NO_SINK + _UNIFORM_BRANCHES,
     "tiled_fma.hip": _NO_SINK + _UNIFORM_BRANCHES,
-    "batch.hip": _NO_SINK,
-    "live.hip": _NO_SINK,
-    "adjoint_tiled.hip": _NO_SINK,
+    "batch.hip": _NO_SINK + _UNIFORM_BRANCHES,
+    "live.hip": _NO_SINK + _UNIFORM_BRANCHES,
+    "adjoint_tiled.hip": _NO_SINK + _UNIFORM_BRANCHES,
     "fused.hip": _NO_SINK,
 }
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall"]

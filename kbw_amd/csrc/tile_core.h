@@ -216,8 +216,8 @@ __device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, const TGate
 // Lane exchanges.  Lane layouts pinned on gfx950:
 //   xor 1, 2:  DPP quad_perm 0xB1 / 0x4E gives lane i the value of lane i^1 / i^2
 //   xor 8:     DPP row_ror:8 (rotate each 16-lane row by 8) gives lane i^8
-//   xor 4:     row_ror:4 gives lane i-4 and row_ror:12 lane i+4 (within the
-//              row); a lane with bit 2 set takes the first
+//   xor 4:     row_half_mirror (lane i^7 within each 8-lane half row), then
+//              quad_perm [3,2,1,0] (lane i^3): two moves, no select
 //   xor 16/32: ds_bpermute.  v_permlane16/32_swap of a register with itself
 //              (then a select on the lane bit) was measured slower: RY on
 //              lane bit 4 or 5 took 510 / 655 us per 30-qubit pass against
@@ -225,7 +225,7 @@ __device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, const TGate
 enum { LX_Q1 = 0, LX_Q2 = 1, LX_R8 = 2, LX_R4 = 3, LX_SHFL = 4 };
 
 template <int LX>
-__device__ __forceinline__ unsigned lane_xor_u(unsigned x, uint32_t lane) {
+__device__ __forceinline__ unsigned lane_xor_u(unsigned x) {
     if constexpr (LX == LX_Q1) {
         return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
     } else if constexpr (LX == LX_Q2) {
@@ -233,20 +233,19 @@ __device__ __forceinline__ unsigned lane_xor_u(unsigned x, uint32_t lane) {
     } else if constexpr (LX == LX_R8) {
         return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, false);
     } else {
-        const unsigned down = (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x124, 0xF, 0xF, false);
-        const unsigned up = (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x12C, 0xF, 0xF, false);
-        return (lane & 4u) ? down : up;
+        const int m7 = __builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, false);
+        return (unsigned)__builtin_amdgcn_mov_dpp(m7, 0x1B, 0xF, 0xF, false);
     }
 }
 
 template <int LX>
-__device__ __forceinline__ double lane_xor_d(double x, uint32_t lane, int lm) {
+__device__ __forceinline__ double lane_xor_d(double x, int lm) {
     if constexpr (LX == LX_SHFL) {
         return __shfl_xor(x, lm, 64);
     } else {
         const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-        const unsigned lo = lane_xor_u<LX>((unsigned)b, lane);
-        const unsigned hi = lane_xor_u<LX>((unsigned)(b >> 32), lane);
+        const unsigned lo = lane_xor_u<LX>((unsigned)b);
+        const unsigned hi = lane_xor_u<LX>((unsigned)(b >> 32));
         return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
     }
 }
@@ -256,11 +255,10 @@ __device__ __forceinline__ void lane_pair(dv2 (&v)[TR], uint32_t tid, uint32_t t
     const uint32_t cm = G.cm;
     const bool tok = thread_ctrl_ok(tid, cm);
     const bool one = ((tid >> tp) & 1u) != 0;
-    const uint32_t lane = tid & 63u;
     const int lm = 1 << tp;
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
-        const dv2 o = mk2(lane_xor_d<LX>(v[r].x, lane, lm), lane_xor_d<LX>(v[r].y, lane, lm));
+        const dv2 o = mk2(lane_xor_d<LX>(v[r].x, lm), lane_xor_d<LX>(v[r].y, lm));
         const dv2 out = pair_val<PK>(v[r], o, one, G);
         if constexpr (CTRL) {
             v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];

@@ -43,18 +43,18 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
     // ng >= 1 (the host launches only non-empty passes); a do-while leaves no
     // path around the loop, so the loaded values need no copy kept for one.
-    // The next gate's record is loaded while the current gate is applied.
+    // Each gate's record is read at the top of its step (scalar loads; the
+    // other resident waves cover their latency).  Prefetching the next record
+    // cost a copy of it through scalar registers on every gate.
     uint32_t i = 0;
-    TGate G = gates[0];
     do {
-        const TGate Gn = gates[i + 1 < ng ? i + 1 : i];
+        const TGate G = gates[i];
         if ((base & G.ocm) == G.ocm) {  // a control outside the tile is 0 here
             if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
                 apply_diag(v, base, tid, G);
             else
                 apply_pair(v, lds, tid, G);
         }
-        G = Gn;
     } while (++i < ng);
     // recompute the store addresses instead of keeping eight of them live
     // through the gate loop

@@ -101,6 +101,14 @@ __device__ __forceinline__ dv2 cmul(dv2 a, dv2 b) {
     return r;
 }
 
+// m = 0 or the sign bit: flips the sign of a double / of both parts when set
+__device__ __forceinline__ double flip_sign_d(double x, uint32_t m) {
+    return __longlong_as_double(__double_as_longlong(x) ^ (long long)((unsigned long long)m << 32));
+}
+__device__ __forceinline__ dv2 flip_sign(dv2 x, uint32_t m) {
+    return mk2(flip_sign_d(x.x, m), flip_sign_d(x.y, m));
+}
+
 __device__ __forceinline__ dv2 shfl_xor2(dv2 v, int mask) {
     return mk2(__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64));
 }
@@ -184,6 +192,21 @@ __device__ __forceinline__ dv2 pair_val(dv2 m, dv2 o, bool one, const TGate &G) 
     }
 }
 
+// pair_val where the half varies across lanes: sg is the sign bit when the
+// amplitude is in the |1> half.  H and Y take their half from sign flips
+// instead of computing both forms and selecting: -m + o is o - m exactly, and
+// (-o.y, o.x) / (o.y, -o.x) are sign flips of the partner.
+template <int PK>
+__device__ __forceinline__ dv2 pair_val_lane(dv2 m, dv2 o, bool one, uint32_t sg, const TGate &G) {
+    if constexpr (PK == PK_Y) {
+        return mk2(flip_sign_d(o.y, sg), flip_sign_d(o.x, sg ^ 0x80000000u));
+    } else if constexpr (PK == PK_H) {
+        return mk2((flip_sign_d(m.x, sg) + o.x) * G.d, (flip_sign_d(m.y, sg) + o.y) * G.d);
+    } else {
+        return pair_val<PK>(m, o, one, G);
+    }
+}
+
 // Target on register bit RB: both amplitudes of a pair live in one thread.
 // CTRL: some control lies inside the tile, so only the pairs whose index
 // matches cm change.
@@ -255,11 +278,12 @@ __device__ __forceinline__ void lane_pair(dv2 (&v)[TR], uint32_t tid, uint32_t t
     const uint32_t cm = G.cm;
     const bool tok = thread_ctrl_ok(tid, cm);
     const bool one = ((tid >> tp) & 1u) != 0;
+    const uint32_t sg = one ? 0x80000000u : 0u;
     const int lm = 1 << tp;
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
         const dv2 o = mk2(lane_xor_d<LX>(v[r].x, lm), lane_xor_d<LX>(v[r].y, lm));
-        const dv2 out = pair_val<PK>(v[r], o, one, G);
+        const dv2 out = pair_val_lane<PK>(v[r], o, one, sg, G);
         if constexpr (CTRL) {
             v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
         } else {
@@ -301,7 +325,7 @@ __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid
 #pragma unroll
         for (int r = 0; r < kChunk; ++r) {
             const dv2 o = lds[pt + (r << TB)];
-            const dv2 out = pair_val<PK>(v[q + r], o, one, G);
+            const dv2 out = pair_val_lane<PK>(v[q + r], o, one, one ? 0x80000000u : 0u, G);
             if constexpr (CTRL) {
                 v[q + r] = slot_ctrl_ok(tok, q + r, cm) ? out : v[q + r];
             } else {
@@ -397,13 +421,6 @@ __device__ __forceinline__ void with_slot_mask(uint32_t m, F &f) {
         else
             with_slot_mask<NB - 1, ACC>(m, f);
     }
-}
-
-// m = 0 or the sign bit: flips the sign of both parts when set
-__device__ __forceinline__ dv2 flip_sign(dv2 x, uint32_t m) {
-    const unsigned long long h = (unsigned long long)m << 32;
-    return mk2(__longlong_as_double(__double_as_longlong(x.x) ^ (long long)h),
-               __longlong_as_double(__double_as_longlong(x.y) ^ (long long)h));
 }
 
 template <int DK>

@@ -39,9 +39,9 @@ METRIC = "30-qubit c128 gates/s + HBM TB/s; adjoint grads/s vs plain at 1e-9"
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE at 30 qubits, each
 # scaled by the same counter on a 1 GiB device copy (fetch x2.0, write x1.0).
-TRAFFIC_PAIR = 34362354613.0
-TRAFFIC_TILE = 34363737074.0
-TRAFFIC_TILE_FMA = 34359724321.0
+TRAFFIC_PAIR = 34362278967.0
+TRAFFIC_TILE = 34367139684.0
+TRAFFIC_TILE_FMA = 34359934142.0
 
 
 def parse():
@@ -588,7 +588,7 @@ def main():
         result["roofline"] = pg["roofline"]
     result["config"] = config
 
-    result["traffic_note"] = "traffic = FETCH_SIZE*1.99993 + WRITE_SIZE*1.0 per launch (scale from a 1 GiB copy)"
+    result["traffic_note"] = "traffic = FETCH_SIZE*1.99992 + WRITE_SIZE*1.0 per launch (scale from a 1 GiB copy)"
     # expectation value of the 64-string Hamiltonian
     if "exp" in legs:
         t1 = time.perf_counter()

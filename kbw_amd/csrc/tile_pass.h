@@ -24,7 +24,14 @@ using namespace tile;
 // PERM: Block swaps folded into the pass; the tile is stored through st, whose
 // qubits are those of ts in another order (position b of the tile goes to
 // qubit st.qbit[b]), so exchanged qubits cost no extra traffic.
-template <bool PERM>
+// Gate records: a state of few tiles (PF) has too few waves to cover a scalar
+// load per gate, so the next record is loaded while the current gate is
+// applied; on a large state each record is read at the top of its step (the
+// other resident waves cover the latency), which saves copying the prefetched
+// record through scalar registers on every gate.
+constexpr uint32_t kPrefetchBelowTiles = 1024;
+
+template <bool PERM, bool PF>
 __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void KBWH_TILE_PASS_NAME(
     dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles,
     TileSpec st) {
@@ -43,19 +50,28 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
     // ng >= 1 (the host launches only non-empty passes); a do-while leaves no
     // path around the loop, so the loaded values need no copy kept for one.
-    // Each gate's record is read at the top of its step (scalar loads; the
-    // other resident waves cover their latency).  Prefetching the next record
-    // cost a copy of it through scalar registers on every gate.
-    uint32_t i = 0;
-    do {
-        const TGate G = gates[i];
+    auto step = [&](const TGate &G) {
         if ((base & G.ocm) == G.ocm) {  // a control outside the tile is 0 here
             if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
                 apply_diag(v, base, tid, G);
             else
                 apply_pair(v, lds, tid, G);
         }
-    } while (++i < ng);
+    };
+    uint32_t i = 0;
+    if constexpr (PF) {
+        TGate G = gates[0];
+        do {
+            const TGate Gn = gates[i + 1 < ng ? i + 1 : i];
+            step(G);
+            G = Gn;
+        } while (++i < ng);
+    } else {
+        do {
+            const TGate G = gates[i];
+            step(G);
+        } while (++i < ng);
+    }
     // recompute the store addresses instead of keeping eight of them live
     // through the gate loop
     if constexpr (PERM) {

@@ -21,8 +21,11 @@ hipError_t launch_tile_contracted(double2 *amp, const void *spec, const void *ga
                   "tile records shared with tiled.hip");
     tile::TileSpec ts;
     memcpy(&ts, spec, sizeof ts);
-    k_tile_fma<false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, (const tile::TGate *)gates, num_gates,
-                                                    ntiles, ts);
+    const tile::TGate *g = (const tile::TGate *)gates;
+    if (ntiles < kPrefetchBelowTiles)
+        k_tile_fma<false, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts);
+    else
+        k_tile_fma<false, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts);
     return hipGetLastError();
 }
 

@@ -17,14 +17,19 @@ hipError_t launch_tile_exact(double2 *amp, const void *spec, const void *store, 
                   "tile records shared with tiled.hip");
     tile::TileSpec ts;
     memcpy(&ts, spec, sizeof ts);
+    const tile::TGate *g = (const tile::TGate *)gates;
+    const bool pf = ntiles < kPrefetchBelowTiles;
     if (store) {
         tile::TileSpec st;
         memcpy(&st, store, sizeof st);
-        k_tile<true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, (const tile::TGate *)gates, num_gates,
-                                                      ntiles, st);
+        if (pf)
+            k_tile<true, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, st);
+        else
+            k_tile<true, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, st);
+    } else if (pf) {
+        k_tile<false, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts);
     } else {
-        k_tile<false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, (const tile::TGate *)gates,
-                                                       num_gates, ntiles, ts);
+        k_tile<false, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts);
     }
     return hipGetLastError();
 }

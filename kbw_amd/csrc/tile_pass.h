@@ -67,10 +67,14 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
             G = Gn;
         } while (++i < ng);
     } else {
+        // walk a record pointer: its fields are then loaded at fixed offsets
+        // from it (an index kept the loads on a pointer into the middle of
+        // the record, rebased with 64-bit adds for every field)
+        const TGate *gp = gates, *const ge = gates + ng;
         do {
-            const TGate G = gates[i];
+            const TGate G = *gp;
             step(G);
-        } while (++i < ng);
+        } while (++gp != ge);
     }
     // recompute the store addresses instead of keeping eight of them live
     // through the gate loop

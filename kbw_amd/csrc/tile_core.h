@@ -182,11 +182,13 @@ __device__ __forceinline__ dv2 pair_val(dv2 m, dv2 o, bool one, const TGate &G) 
         const double sy = one ? o.y - m.y : m.y + o.y;
         return mk2(sx * G.d, sy * G.d);
     } else if constexpr (PK == PK_TWO) {
-        const PairCoef k = pair_coef(G, one ? 1.0 : -1.0);
-        return mk2(k.A * m.x + k.B * o.x, k.A * m.y + k.B * o.y);
+        // RY's record has as = bu = 0 (set_coefficients): A = au + as sigma is
+        // au and B = bu + bs sigma is sigma bs, exactly
+        const double B = one ? G.bs : -G.bs;
+        return mk2(G.au * m.x + B * o.x, G.au * m.y + B * o.y);
     } else if constexpr (PK == PK_RX) {
-        const PairCoef k = pair_coef(G, one ? 1.0 : -1.0);
-        return mk2(k.A * m.x + k.Cr * o.y, k.A * m.y + k.Cr * -o.x);
+        // RX's record has as = crs = 0: A = au and Cr = cru for both halves
+        return mk2(G.au * m.x + G.cru * o.y, G.au * m.y + G.cru * -o.x);
     } else {
         return pair_out(m, o, pair_coef(G, one ? 1.0 : -1.0));
     }
@@ -202,6 +204,9 @@ __device__ __forceinline__ dv2 pair_val_lane(dv2 m, dv2 o, bool one, uint32_t sg
         return mk2(flip_sign_d(o.y, sg), flip_sign_d(o.x, sg ^ 0x80000000u));
     } else if constexpr (PK == PK_H) {
         return mk2((flip_sign_d(m.x, sg) + o.x) * G.d, (flip_sign_d(m.y, sg) + o.y) * G.d);
+    } else if constexpr (PK == PK_TWO) {
+        const double B = flip_sign_d(G.bs, sg ^ 0x80000000u);
+        return mk2(G.au * m.x + B * o.x, G.au * m.y + B * o.y);
     } else {
         return pair_val<PK>(m, o, one, G);
     }

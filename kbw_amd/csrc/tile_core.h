@@ -72,7 +72,9 @@ struct TGate {
     double au, as, bu, bs, cru, crs, ciu, cis, d;
     // diagonal gates: the phases of the |0> and |1> halves
     double ph0x, ph0y, ph1x, ph1y;
-    double pad2;
+    // diagonal gates: the thread bits and the register-slot bits that must be
+    // 1 (Z and P: target and controls; RZ: controls), set by make_tgate
+    uint32_t dlo, dhi;
 };
 static_assert(sizeof(TGate) == 144, "TGate layout");
 
@@ -436,11 +438,11 @@ __device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32
     const bool reg_t = rb < 8u;         // target on a register bit
     const bool out_t = tp == kOutside;  // target outside the tile: its bit is fixed per tile
     const bool thr_t = !reg_t && !out_t;
-    const uint32_t clo = cm & (uint32_t)(TT - 1), chi = cm >> TB;
+    const uint32_t clo = G.dlo, chi = G.dhi;
     if constexpr (DK == DK_Z || DK == DK_P) {
-        if (out_t && (base & G.otb) == 0) return;
-        const uint32_t mlo = clo | (thr_t ? 1u << tp : 0u);
-        const uint32_t mhi = chi | (reg_t ? 1u << rb : 0u);
+        // dlo / dhi hold target and controls; a target outside the tile is
+        // among the record's outside controls, tested before the gate
+        const uint32_t mlo = clo, mhi = chi;
         const bool sel = (tid & mlo) == mlo;
         if constexpr (DK == DK_Z) {
             const uint32_t m = sel ? 0x80000000u : 0u;
@@ -617,6 +619,20 @@ inline TGate make_tgate(const PhysOp &o, const int pos_of[64], uint32_t n) {
             g.ocm |= 1ull << q;
     }
     set_coefficients(g, o.kind, o.angle);
+    if (is_diagonal(o.kind)) {
+        g.dlo = g.cm & (uint32_t)(TT - 1);
+        g.dhi = g.cm >> TB;
+        if (o.kind != KRZ) {
+            // Z and P change the amplitudes whose target and controls are all
+            // 1: a target outside the tile acts as one more outside control
+            if (g.tpos == kOutside)
+                g.ocm |= g.otb;
+            else if (g.tpos < (uint32_t)TB)
+                g.dlo |= 1u << g.tpos;
+            else
+                g.dhi |= 1u << (g.tpos - TB);
+        }
+    }
     return g;
 }
 

@@ -41,6 +41,7 @@ HIP_SOURCE_FLAGS = {
     "live.hip": _NO_SINK + _UNIFORM_BRANCHES,
     "adjoint_tiled.hip": _NO_SINK + _UNIFORM_BRANCHES,
     "fused.hip": _NO_SINK,
+    "tiled.hip": _NO_SINK + _UNIFORM_BRANCHES,
 }
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 

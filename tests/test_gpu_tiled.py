@@ -83,6 +83,52 @@ def test_tiled_every_kind_at_every_tile_position(kind):
         b.close()
 
 
+def test_tiled_diagonal_masks_in_one_pass():
+    """Z, P and RZ with their target outside the tile, on register bits and on
+    lane and wave bits, under control sets that mix lane, register, wave and
+    outside bits, all in one pass (the gate records carry the thread and
+    register-slot masks of target and controls; a Z or P target outside the
+    tile joins the outside controls).  X gates on qubits 3..10 fix the tile
+    positions as in the every-position test: 3, 4, 5 -> register bits 8, 9,
+    10; 6, 7, 8 -> lane bits 3, 4, 5; 9, 10 -> wave bits 6, 7; qubits 11 and
+    12 stay outside.  Tiled runs are bit-exact with the per-gate path;
+    dependency-ordered runs agree to 1e-12."""
+    from kbw_amd import RUN_REORDER
+
+    print("tiled diagonal masks", flush=True)
+    n = 13
+    rng = np.random.default_rng(77)
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    st /= np.linalg.norm(st)
+    cases = [(11, []), (11, [0]), (11, [3]), (11, [7]), (11, [9]), (11, [12]), (11, [0, 3, 12]),
+             (3, []), (3, [11]), (3, [0, 12]), (4, [1, 5]), (5, [8, 10]),
+             (0, []), (0, [3, 4]), (1, [11, 12]), (7, [2, 9]), (9, []), (10, [3, 6, 11]), (12, [4, 10])]
+    ops = circuits.OpList()
+    for t in range(3, 11):
+        ops.add(0, t)
+    for kind in (2, 4, 7):
+        for t, ctrl in cases:
+            ops.add(kind, t, ctrl, angle=0.3 + 0.1 * t)
+    arr = ops.array()
+    ref = DenseHip(n)
+    ref.set_state(st)
+    ref.run(arr, mode=RUN_PER_GATE)
+    want = ref.state()
+    ref.close()
+    a = DenseHip(n)
+    a.set_state(st)
+    a.run(arr, mode=RUN_TILED)
+    assert a.run_stats()[1] == 1
+    bad = np.nonzero(a.state() != want)[0]
+    assert bad.size == 0, bad[:8]
+    a.close()
+    b = DenseHip(n)
+    b.set_state(st)
+    b.run(arr, mode=RUN_REORDER)
+    assert np.max(np.abs(b.state() - want)) <= 1e-12
+    b.close()
+
+
 def test_tiled_small_state_uses_per_gate():
     n = 8
     ops = circuits.random_circuit(np.random.default_rng(3), n, 100)

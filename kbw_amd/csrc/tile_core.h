@@ -534,7 +534,76 @@ struct PhysOp {
     uint32_t target;
     uint64_t ctrl;  // physical control mask without the target bit
     double angle;
+    // a fused diagonal gate carries its phases {|0> re, im, |1> re, im}
+    bool phased = false;
+    double ph[4] = {0.0, 0.0, 0.0, 0.0};
 };
+
+// The phases of an uncontrolled diagonal op, as set_coefficients gives them.
+inline void diag_phases(const PhysOp &o, double ph[4]) {
+    if (o.phased) {
+        memcpy(ph, o.ph, sizeof o.ph);
+        return;
+    }
+    const Coef k = make_coef(o.kind, o.angle);
+    if (o.kind == KZ) {
+        ph[0] = 1.0, ph[1] = 0.0, ph[2] = -1.0, ph[3] = -0.0;
+    } else if (o.kind == KP) {
+        ph[0] = 1.0, ph[1] = 0.0, ph[2] = k.ph1.x, ph[3] = k.ph1.y;
+    } else {
+        ph[0] = k.ph0.x, ph[1] = k.ph0.y, ph[2] = k.ph1.x, ph[3] = k.ph1.y;
+    }
+}
+
+// Fuses uncontrolled diagonal gates (dependency-ordered runs only: the fused
+// phases are products of the gates' phases, equal to rounding).  A Z, P or RZ
+// on qubit q merges into the last earlier uncontrolled diagonal gate on q
+// when every op between them that touches q is diagonal on q: a diagonal gate
+// on q, or a gate with q among its controls; those commute with it.  The
+// result is a P when its |0> phase is exactly 1, nothing when both phases
+// are exactly 1, else a general diagonal (the RZ form with its own phases).
+inline void fuse_diagonals(std::vector<PhysOp> &ops, uint32_t n) {
+    std::vector<PhysOp> out;
+    out.reserve(ops.size());
+    std::vector<std::vector<uint32_t>> touch(n);  // ops of out touching qubit q
+    for (const PhysOp &o : ops) {
+        if (is_diagonal(o.kind) && o.ctrl == 0) {
+            const std::vector<uint32_t> &lst = touch[o.target];
+            int64_t j = -1;
+            for (auto it = lst.rbegin(); it != lst.rend(); ++it) {
+                const PhysOp &p = out[*it];
+                if (p.kind < 0) continue;  // fused away
+                if (is_diagonal(p.kind) && p.ctrl == 0 && p.target == o.target) {
+                    j = (int64_t)*it;
+                    break;
+                }
+                if (!is_diagonal(p.kind) && p.target == o.target) break;
+            }
+            if (j >= 0) {
+                PhysOp &m = out[(size_t)j];
+                double a[4], b[4], r[4];
+                diag_phases(m, a);
+                diag_phases(o, b);
+                r[0] = a[0] * b[0] - a[1] * b[1];
+                r[1] = a[0] * b[1] + a[1] * b[0];
+                r[2] = a[2] * b[2] - a[3] * b[3];
+                r[3] = a[2] * b[3] + a[3] * b[2];
+                const bool one0 = r[0] == 1.0 && r[1] == 0.0;
+                m.kind = one0 && r[2] == 1.0 && r[3] == 0.0 ? -1 : (one0 ? KP : KRZ);
+                m.phased = true;
+                memcpy(m.ph, r, sizeof r);
+                continue;
+            }
+        }
+        out.push_back(o);
+        const uint32_t idx = (uint32_t)(out.size() - 1);
+        touch[o.target].push_back(idx);
+        for (uint64_t c = o.ctrl; c; c &= c - 1) touch[__builtin_ctzll(c)].push_back(idx);
+    }
+    ops.clear();
+    for (const PhysOp &o : out)
+        if (o.kind >= 0) ops.push_back(o);
+}
 
 struct Pass {
     std::vector<PhysOp> ops;
@@ -619,6 +688,12 @@ inline TGate make_tgate(const PhysOp &o, const int pos_of[64], uint32_t n) {
             g.ocm |= 1ull << q;
     }
     set_coefficients(g, o.kind, o.angle);
+    if (o.phased) {
+        g.ph0x = o.ph[0];
+        g.ph0y = o.ph[1];
+        g.ph1x = o.ph[2];
+        g.ph1y = o.ph[3];
+    }
     if (is_diagonal(o.kind)) {
         g.dlo = g.cm & (uint32_t)(TT - 1);
         g.dhi = g.cm >> TB;

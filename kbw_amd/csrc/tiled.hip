@@ -521,6 +521,7 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
             const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
             phys[i] = PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), angle};
         }
+        fuse_diagonals(phys, n);
         std::vector<uint32_t> order, ends;
         schedule_passes(phys, n, cap, order, ends);
         size_t k = 0;

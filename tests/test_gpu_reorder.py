@@ -36,6 +36,53 @@ def test_reorder_random_circuits(n, ngates, seed):
     assert np.max(np.abs(g.state() - o.state())) < TOL
 
 
+@pytest.mark.parametrize("n,seed", [(12, 0), (15, 1), (19, 2)])
+def test_reorder_fused_diagonals(n, seed):
+    """Chains of uncontrolled Z, P and RZ on each qubit, interleaved with CZ,
+    controlled phases and CNOTs whose controls sit on those qubits (diagonal
+    gates fuse across them), with H and RY (X-type on their target: fusion
+    stops there) and pairs that cancel exactly (Z Z, P(a) P(-a) on the |1>
+    half): the dependency-ordered run agrees with the oracle to 1e-12."""
+    print(f"reorder fused diagonals n={n}", flush=True)
+    rng = np.random.default_rng(seed + 3000)
+    ops = circuits.OpList()
+    for layer in range(12):
+        for q in range(n):
+            r = int(rng.integers(0, 6))
+            a = float(rng.uniform(-np.pi, np.pi))
+            if r == 0:
+                ops.add(2, q)
+                ops.add(2, q)  # cancels
+            elif r == 1:
+                ops.add(4, q, angle=a)
+                ops.add(4, q, angle=-a)
+            elif r == 2:
+                ops.add(7, q, angle=a)
+                ops.add(4, q, angle=0.5 * a)
+            elif r == 3:
+                ops.add(4, q, angle=a)
+            elif r == 4:
+                ops.add(7, q, angle=a)
+            else:
+                ops.add(2, q)
+        for q in range(layer % 2, n - 1, 2):
+            c = int(rng.integers(0, 3))
+            if c == 0:
+                ops.add(2, q + 1, [q])  # CZ
+            elif c == 1:
+                ops.add(4, q + 1, [q], angle=float(rng.uniform(-np.pi, np.pi)))  # controlled phase
+            else:
+                ops.add(0, q + 1, [q])  # CNOT: fusion continues on q, stops on q + 1
+        if layer % 3 == 2:
+            for q in range(0, n, 3):
+                ops.add(3 if q % 2 else 6, q, angle=0.7)
+    arr = ops.array()
+    g, _ = passes_of(n, arr, RUN_REORDER)
+    o = oracle.DenseV2(n)
+    o.run(arr)
+    assert np.max(np.abs(g.state() - o.state())) < TOL
+
+
 @pytest.mark.parametrize("kind,n", [("brickwork", 20), ("hea", 18)])
 def test_reorder_layered_circuits(kind, n):
     print(f"reorder {kind} n={n}", flush=True)

@@ -68,8 +68,10 @@ extern "C" {
 #define KBWH_RUN_FUSED 2u    /* gates fused into k-qubit blocks applied with FP64 MFMA; equal to rounding (~1e-16 per block), not bit for bit */
 #define KBWH_RUN_AUTO 3u     /* the configuration chosen by kbwh_autotune for this handle (KBWH_RUN_TILED until tuned) */
 #define KBWH_RUN_REORDER 4u  /* tiled passes over a dependency order of the gates (gates on disjoint qubits may trade places):
-                                the same unitary, amplitudes equal to rounding; fewer passes.  A Block state keeps the reference's
-                                map and swap count (decided in program order); its swaps are ordered with the gates */
+                                the same unitary, amplitudes equal to rounding; fewer passes.  On a flat state, uncontrolled
+                                Z/P/RZ gates on one qubit are first fused across the gates diagonal on that qubit (phase
+                                products).  A Block state keeps the reference's map and swap count (decided in program
+                                order); its swaps are ordered with the gates */
 
 /* One circuit operation (24 bytes).  `angle` is used unless `param` >= 0
  * and a parameter vector is supplied, in which case the angle is
@@ -120,7 +122,8 @@ int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_
  * on the host only (no device needed; no counterpart in the reference):
  * order[0..num_ops) receives the op indices in execution order and
  * pass_end[0..*num_passes) the end of each pass in `order` (pass_end needs
- * room for num_ops entries).  tile_targets as in kbwh_set_run_config. */
+ * room for num_ops entries).  tile_targets as in kbwh_set_run_config.
+ * The plan covers the ops as given: a run fuses diagonal gates first. */
 int32_t kbwh_reorder_plan(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits, uint32_t tile_targets,
                           uint32_t *order, uint32_t *pass_end, uint64_t *num_passes);
 

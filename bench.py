@@ -39,9 +39,9 @@ METRIC = "30-qubit c128 gates/s + HBM TB/s; adjoint grads/s vs plain at 1e-9"
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE at 30 qubits, each
 # scaled by the same counter on a 1 GiB device copy (fetch x2.0, write x1.0).
-TRAFFIC_PAIR = 34362347975.0
-TRAFFIC_TILE = 34368735077.0
-TRAFFIC_TILE_FMA = 34359949918.0
+TRAFFIC_PAIR = 34362315175.0
+TRAFFIC_TILE = 34369246667.0
+TRAFFIC_TILE_FMA = 34359974941.0
 
 
 def parse():

@@ -305,12 +305,29 @@ __global__ __launch_bounds__(kBlock) void k_hier_sample(const double2 *__restric
     double base = 0.0;
     uint64_t start = 0, cnt = lv.count[top], pick = 0;
     for (int l = top; l >= 0; --l) {
-        double acc = base;
-        uint64_t e = 0;
+        // the first child whose running sum (from the parent's base) exceeds
+        // x; else the last child of nonzero weight.  The parent summed its
+        // children from 0, so rounding can leave x past every running sum
+        // here, and a trailing child of weight 0 must not be picked.
+        double acc = base, acc_nz = base;
+        uint64_t e = 0, nz = cnt;
+        bool hit = false;
         for (; e + 1 < cnt; ++e) {
-            const double nxt = acc + level_value(a, lv, l, start + e);
-            if (x < nxt) break;
+            const double w = level_value(a, lv, l, start + e);
+            const double nxt = acc + w;
+            if (x < nxt) {
+                hit = true;
+                break;
+            }
+            if (w > 0.0) {
+                nz = e;
+                acc_nz = acc;
+            }
             acc = nxt;
+        }
+        if (!hit && !(level_value(a, lv, l, start + cnt - 1) > 0.0) && nz < cnt) {
+            e = nz;
+            acc = acc_nz;
         }
         base = acc;
         pick = start + e;

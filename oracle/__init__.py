@@ -38,6 +38,7 @@ _SIG = {
     "orc_v1_measure_collapse": ([_vp, _u32, _i32, _f64], None),
     "orc_uniform_scale": ([_f64], _f64), "orc_measure": ([_vp, _p, _u32, _p], _u64),
     "orc_exp_value": ([_vp, _p, _p, _p, _p, _u32], _f64),
+    "orc_exp_value_chunked": ([_vp, _p, _p, _p, _p, _u32], _f64),
     "orc_sample_sequential": ([_vp, _p, _u32, _u64, _p, _p], _i32),
     "orc_sample_hier": ([_p, _u32, _p, _u32, _u64, _p, _p], _i32),
     "orc_block_new": ([_u32, _u32], _vp), "orc_block_free": ([_vp], None),
@@ -184,6 +185,12 @@ class DenseV2:
     def exp_value(self, pauli_strings, coefficients):
         pa, qu, off, co = flatten_hamiltonian(pauli_strings, coefficients)
         return self._l.orc_exp_value(self._s, _ptr(pa), _ptr(qu), _ptr(off), _ptr(co), len(co))
+
+    def exp_value_chunked(self, pauli_strings, coefficients):
+        """exp_value with each string's sum split into 2^20-amplitude chunks
+        summed in parallel (same bits as exp_value for n <= 20)."""
+        pa, qu, off, co = flatten_hamiltonian(pauli_strings, coefficients)
+        return self._l.orc_exp_value_chunked(self._s, _ptr(pa), _ptr(qu), _ptr(off), _ptr(co), len(co))
 
     def sample_sequential(self, qubits, shots, uniforms):
         q = _u32(qubits)

@@ -84,6 +84,10 @@ double orc_uniform_scale(double total);
 uint64_t orc_measure(orc_v2 *s, const uint32_t *qubits, uint32_t nq, const double *uniforms);
 double orc_exp_value(orc_v2 *s, const uint8_t *paulis, const uint32_t *qubits,
                      const uint32_t *offsets, const double *coeffs, uint32_t nstrings);
+/* The same terms summed in 2^20-position chunks (parallel), then the chunk
+ * partials in order: equal bits for n <= 20, rounding-close above. */
+double orc_exp_value_chunked(orc_v2 *s, const uint8_t *paulis, const uint32_t *qubits,
+                             const uint32_t *offsets, const double *coeffs, uint32_t nstrings);
 /* Sampling with the faithful sequential cumulative sum over dump entries. */
 int orc_sample_sequential(orc_v2 *s, const uint32_t *qubits, uint32_t nq, uint64_t shots,
                           const double *uniforms, uint64_t *outcomes);

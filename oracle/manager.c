@@ -109,6 +109,51 @@ double orc_exp_value(orc_v2 *s, const uint8_t *paulis, const uint32_t *qubits,
 }
 
 /*
+ * orc_exp_value with the per-string sum split into chunks of 2^20 dump
+ * positions: each chunk is summed sequentially in ascending index order (the
+ * terms of orc_exp_value, unchanged), the chunks in parallel, and the chunk
+ * partials sequentially in ascending order.  Only the association of the
+ * additions differs from the single running sum of from_dump_to_prob
+ * (convert.rs:30-42); for n <= 20 there is one chunk and the result is the
+ * same bits.  It lets large states (30 qubits) be checked in seconds.
+ */
+#define EXP_CHUNK_BITS 20u
+
+double orc_exp_value_chunked(orc_v2 *s, const uint8_t *paulis, const uint32_t *qubits,
+                             const uint32_t *offsets, const double *coeffs, uint32_t nstrings) {
+    const orc_c64 *a = orc_v2_data(s);
+    const uint32_t n = orc_v2_num_qubits(s);
+    const uint64_t size = (uint64_t)1 << n;
+    const uint64_t chunk = n > EXP_CHUNK_BITS ? ((uint64_t)1 << EXP_CHUNK_BITS) : size;
+    const uint64_t nchunks = size / chunk;
+    double *part = (double *)malloc(nchunks * sizeof(double));
+    double total = 0.0;
+    for (uint32_t k = 0; k < nstrings; k++) {
+        const uint32_t lo = offsets[k], hi = offsets[k + 1];
+        basis_change(s, paulis, qubits, lo, hi, 0);
+#pragma omp parallel for schedule(static) if (nchunks > 1)
+        for (uint64_t c = 0; c < nchunks; c++) {
+            double r = 0.0;
+            for (uint64_t i = c * chunk; i < (c + 1) * chunk; i++) {
+                const double h = hypot(a[i].re, a[i].im);
+                if (!(h > ORC_SMALL_EPSILON)) continue;
+                const double prob = h * h;
+                const uint64_t key = orc_dump_key(i, qubits + lo, hi - lo);
+                const double parity = (__builtin_popcountll(key) % 2 == 0) ? 1.0 : -1.0;
+                r += prob * parity;
+            }
+            part[c] = r;
+        }
+        double result = 0.0;
+        for (uint64_t c = 0; c < nchunks; c++) result += part[c];
+        basis_change(s, paulis, qubits, lo, hi, 1);
+        total += result * coeffs[k];
+    }
+    free(part);
+    return total;
+}
+
+/*
  * Default sample (quantum_execution.rs:99-102) through from_dump_to_prob and
  * from_prob_to_shots (convert.rs:30-73) with rand's WeightedIndex: weights in
  * dump order, cumulative_weights[j] = w_0 + ... + w_j for j < m-1 (one
@@ -176,7 +221,12 @@ int orc_sample_sequential(orc_v2 *s, const uint32_t *qubits, uint32_t nq, uint64
  *   x       = u * scale(total)  (rand's UniformFloat, as in orc_measure);
  *   descent = at each level, starting from the parent's running base, scan
  *             the 64 children: pick the first child e with x < base + T_e
- *             (base += T_e otherwise), or the last child.
+ *             (base += T_e otherwise), or else the last child of nonzero
+ *             weight (with the running base before it).  The child level
+ *             sums from the parent's base, the parent summed its children
+ *             from 0, so rounding can leave x past every child's running sum
+ *             although the parent was picked; the last child may then have
+ *             weight 0, an entry the dump filter of the reference excludes.
  * The picked index is the physical amplitude index; the outcome is its dump
  * key.  The reference's rule (first entry whose running sum exceeds x, else
  * the last) is kept; only the association of the additions changes.
@@ -229,12 +279,25 @@ int orc_sample_hier(const orc_c64 *amp, uint32_t num_qubits, const uint32_t *qub
             uint64_t pick = 0;
             for (int l = top; l >= 0; l--) {
                 const double *v = levels[l];
-                uint64_t e = 0;
-                double acc = base;
+                uint64_t e = 0, nz = cnt;
+                double acc = base, acc_nz = base;
+                int hit = 0;
                 for (e = 0; e + 1 < cnt; e++) {
-                    const double nxt = acc + v[start + e];
-                    if (x < nxt) break;
+                    const double w = v[start + e];
+                    const double nxt = acc + w;
+                    if (x < nxt) {
+                        hit = 1;
+                        break;
+                    }
+                    if (w > 0.0) {
+                        nz = e;
+                        acc_nz = acc;
+                    }
                     acc = nxt;
+                }
+                if (!hit && !(v[start + cnt - 1] > 0.0) && nz < cnt) {
+                    e = nz;
+                    acc = acc_nz;
                 }
                 base = acc;
                 pick = start + e;

@@ -212,6 +212,14 @@ class DenseV2:
         buf = (C.c_double * (2 * n)).from_address(self._l.orc_v2_data(self._s))
         return np.frombuffer(buf, dtype=np.complex128).copy()
 
+    def state_view(self, first, count):
+        """Amplitudes [first, first+count) as a view of the oracle's buffer
+        (no copy; valid while this object lives and the state is unchanged)."""
+        assert 0 <= first and first + count <= (1 << self.n)
+        addr = self._l.orc_v2_data(self._s) + 16 * first
+        buf = (C.c_double * (2 * count)).from_address(addr)
+        return np.frombuffer(buf, dtype=np.complex128)
+
     def set_state(self, amps):
         n = 1 << self.n
         a = np.ascontiguousarray(amps, dtype=np.complex128)

@@ -178,6 +178,22 @@ int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qub
                        const uint32_t *offsets, const double *coeffs, uint32_t num_strings,
                        double *out);
 
+/* The same value as kbwh_exp_value, computed without allocating memory (no
+ * counterpart in the reference).  For callers that cannot allocate: code
+ * recorded into a HIP graph, tight serving loops, hosts that forbid device
+ * allocation after start-up.  kbwh_exp_value may grow the handle's table
+ * buffer with hipMalloc; this variant keeps its block partials, per-string
+ * results and string table in one fixed device array per GPU (about 160 KiB,
+ * held until the process ends).  Limits: at most 4096 strings
+ * (KBWH_ERR_CAPACITY otherwise); no queued gates pending
+ * (KBWH_ERR_INVALID_ARGUMENT: call kbwh_synchronize first, since a flush may
+ * allocate).  All calls on one GPU share the array: calls that overlap in
+ * time on one GPU (two threads, two handles) race and must be serialised by
+ * the caller.  The result bits equal kbwh_exp_value's. */
+int32_t kbwh_exp_value_noalloc(kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
+                               const uint32_t *offsets, const double *coeffs, uint32_t num_strings,
+                               double *out);
+
 /* ---- adjoint differentiation (no counterpart in the reference: its
  * BatchExecution::gradient returns NativeGateUnsupported,
  * quantum_execution.rs:420-426, and libket falls back to parameter shift) --

@@ -57,6 +57,7 @@ SIGNATURES = {
     "kbwh_dump": [_vp, _p, _u32, _u64, C.POINTER(_u64), _p, _p, _p],
     "kbwh_sample": [_vp, _p, _u32, _u64, _p, _p],
     "kbwh_exp_value": [_vp, _p, _p, _p, _p, _u32, C.POINTER(_f64)],
+    "kbwh_exp_value_noalloc": [_vp, _p, _p, _p, _p, _u32, C.POINTER(_f64)],
     "kbwh_adjoint_gradient": [_vp, _p, _u64, _p, _u32, _p, _p, _p, _p, _u32, C.POINTER(_f64), _p],
     "kbwh_read_amplitudes": [_vp, _u64, _u64, _p],
     "kbwh_write_amplitudes": [_vp, _u64, _u64, _p],

@@ -2,6 +2,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "state.h"
@@ -66,18 +67,15 @@ int32_t replay_basis_touches(kbwh_state *s, const uint8_t *paulis, const uint32_
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
-                       const uint32_t *offsets, const double *coeffs, uint32_t ns, double *out) {
-    if (!s || !out || (ns && (!offsets || !coeffs)))
-        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
-    FLUSH(s);
-    if (ns && offsets[ns] > offsets[0] && (!paulis || !qubits))
-        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null Pauli arrays");
-    for (uint32_t k = 0; k < ns; ++k)
-        if (offsets[k + 1] < offsets[k])
-            return set_error(KBWH_ERR_INVALID_ARGUMENT, "offsets not ascending");
+// kbwh_exp_value after the argument checks and the queue flush.  work: device
+// memory to use instead of the handle's reduction area and gate buffer
+// (kbwh_exp_value_noalloc), or null.
+int32_t exp_value_impl(kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits, const uint32_t *offsets,
+                       const double *coeffs, uint32_t ns, double *out, const ExpWork *work) {
+    double *partial = work ? work->partial : s->partial;
+    double *results = work ? work->results : s->result;
     if (!s->block && s->n >= 11 && ns > 0) {
         // grouped read-only sweeps, several strings per read of the state; a
         // string with more than 8 X/Y qubits above qubit 2 is summed alone
@@ -97,17 +95,17 @@ int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qub
         }
         double total = 0.0;
         if (!grouped.empty()) {
-            int32_t rc = exp_value_tiled(s, grouped, gc.data(), &total);
+            int32_t rc = exp_value_tiled(s, grouped, gc.data(), &total, work);
             if (rc) return rc;
         }
         for (size_t k = 0; k < alone.size(); ++k) {
             uint32_t nb = 0;
-            hipError_t e = launch_pauli_partial(s->amp, s->n, alone[k].x, alone[k].z, alone[k].ny, s->partial, &nb,
+            hipError_t e = launch_pauli_partial(s->amp, s->n, alone[k].x, alone[k].z, alone[k].ny, partial, &nb,
                                                 s->stream);
-            if (e == hipSuccess) e = launch_sum_final(s->partial, nb, s->result, s->stream);
+            if (e == hipSuccess) e = launch_sum_final(partial, nb, results, s->stream);
             if (e != hipSuccess) return hip_status(e, "exp_value");
             double v = 0.0;
-            int32_t rc = reduce_to_host(s, &v, 1);
+            int32_t rc = reduce_from(s, results, &v, 1);
             if (rc) return rc;
             total += v * ac[k];
         }
@@ -123,12 +121,12 @@ int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qub
         PauliMasks m;
         if ((rc = pauli_masks(s, paulis, qubits, lo, hi, true, &m))) return rc;
         uint32_t nb = 0;
-        hipError_t e = launch_pauli_partial(s->amp, s->n, m.x, m.z, m.ny, s->partial, &nb, s->stream);
-        if (e == hipSuccess) e = launch_sum_final(s->partial, nb, s->result + pending, s->stream);
+        hipError_t e = launch_pauli_partial(s->amp, s->n, m.x, m.z, m.ny, partial, &nb, s->stream);
+        if (e == hipSuccess) e = launch_sum_final(partial, nb, results + pending, s->stream);
         if (e != hipSuccess) return hip_status(e, "exp_value");
         if (s->block && (rc = replay_basis_touches(s, paulis, qubits, lo, hi))) return rc;
         if (++pending == kResultSlots || k + 1 == ns) {
-            if ((rc = reduce_to_host(s, values.data() + first, pending))) return rc;
+            if ((rc = reduce_from(s, results, values.data() + first, pending))) return rc;
             first += pending;
             pending = 0;
         }
@@ -138,6 +136,85 @@ int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qub
     for (uint32_t k = 0; k < ns; ++k) total += values[k] * coeffs[k];
     *out = total;
     return KBWH_OK;
+}
+
+int32_t exp_value_checks(kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits, const uint32_t *offsets,
+                         const double *coeffs, uint32_t ns, double *out) {
+    if (!s || !out || (ns && (!offsets || !coeffs)))
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    if (ns && offsets[ns] > offsets[0] && (!paulis || !qubits))
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null Pauli arrays");
+    for (uint32_t k = 0; k < ns; ++k)
+        if (offsets[k + 1] < offsets[k])
+            return set_error(KBWH_ERR_INVALID_ARGUMENT, "offsets not ascending");
+    return KBWH_OK;
+}
+
+// ---- the allocation-free variant's work area ------------------------------
+// One fixed array per GPU, held until the process ends: block partials, then
+// per-group / per-string results, then the string table of at most
+// kNoAllocStrings records.  Every kernel writes each slot it later reads
+// within the same call, so nothing needs clearing between calls.
+constexpr uint32_t kNoAllocStrings = 4096;
+constexpr size_t kNoAllocDoubles = kExpPartials + kResultSlots + kNoAllocStrings * kExpStringBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices];
+
+// The device address of `scratch` on `device` (each GPU holds its own copy),
+// looked up once per device with that device current.  The array's name is
+// only a host-side stand-in; kernels get this address.
+int32_t scratch_on(int device, double **out) {
+    if (device < 0 || device >= kMaxDevices) return set_error(KBWH_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+        void *p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) {
+            const hipError_t r = hipSetDevice(cur);
+            if (e == hipSuccess) e = r;
+        }
+        if (e != hipSuccess) return hip_status(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return set_error(KBWH_ERR_DEVICE, "no device address for the exp_value work area");
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return KBWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
+                       const uint32_t *offsets, const double *coeffs, uint32_t ns, double *out) {
+    int32_t rc = exp_value_checks(s, paulis, qubits, offsets, coeffs, ns, out);
+    if (rc) return rc;
+    FLUSH(s);
+    return exp_value_impl(s, paulis, qubits, offsets, coeffs, ns, out, nullptr);
+}
+
+int32_t kbwh_exp_value_noalloc(kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
+                               const uint32_t *offsets, const double *coeffs, uint32_t ns, double *out) {
+    int32_t rc = exp_value_checks(s, paulis, qubits, offsets, coeffs, ns, out);
+    if (rc) return rc;
+    if (ns > kNoAllocStrings) return set_error(KBWH_ERR_CAPACITY, "kbwh_exp_value_noalloc takes at most 4096 strings");
+    // flushing queued gates may grow the gate buffer
+    if (!s->queue.empty())
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "queued gates pending: call kbwh_synchronize before kbwh_exp_value_noalloc");
+    double *base = nullptr;
+    if ((rc = scratch_on(s->device, &base))) return rc;
+    ExpWork w;
+    w.partial = base;
+    w.results = base + kExpPartials;
+    w.strings = base + kExpPartials + kResultSlots;
+    w.strings_cap = kNoAllocStrings;
+    return exp_value_impl(s, paulis, qubits, offsets, coeffs, ns, out, &w);
 }
 
 int32_t kbwh_dump(kbwh_state *s, const uint32_t *qubits, uint32_t nq, uint64_t capacity,

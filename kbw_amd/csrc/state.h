@@ -96,7 +96,22 @@ struct PauliMasks {
     uint64_t x, z;
     uint32_t ny;
 };
-int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out);
+// Device memory an exp_value evaluation may use instead of the handle's own
+// buffers (which grow with hipMalloc): a string table of strings_cap records
+// of kExpStringBytes, block partials (kExpPartials doubles) and per-group or
+// per-string results (kResultSlots doubles).
+constexpr size_t kExpStringBytes = 24, kExpPartials = 4096 + 64;
+struct ExpWork {
+    void *strings;
+    size_t strings_cap;
+    double *partial;
+    double *results;
+};
+int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out,
+                        const ExpWork *work = nullptr);
+// Copies count doubles of device results (stream-ordered) to the host through
+// the handle's pinned buffer.
+int32_t reduce_from(kbwh_state *s, const double *src, double *out, uint32_t count);
 int32_t pauli_apply_tiled(kbwh_state *s, const double2 *phi, double2 *lam, const std::vector<PauliMasks> &ms,
                           const double *coeffs);
 int32_t pauli_masks(const kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,

@@ -136,13 +136,15 @@ int32_t apply_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint6
     return KBWH_OK;
 }
 
-int32_t reduce_to_host(kbwh_state *s, double *out, uint32_t count) {
-    HIPCHK(hipMemcpyAsync(s->host_result, s->result, count * sizeof(double), hipMemcpyDeviceToHost,
-                          s->stream));
+int32_t reduce_from(kbwh_state *s, const double *src, double *out, uint32_t count) {
+    if (count > kResultSlots) return fail(KBWH_ERR_CAPACITY, "more results than the host buffer holds");
+    HIPCHK(hipMemcpyAsync(s->host_result, src, count * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     memcpy(out, s->host_result, count * sizeof(double));
     return KBWH_OK;
 }
+
+int32_t reduce_to_host(kbwh_state *s, double *out, uint32_t count) { return reduce_from(s, s->result, out, count); }
 
 int32_t measure_p1(kbwh_state *s, uint32_t target, double *p1) {
     if (target >= s->n) return fail(KBWH_ERR_INVALID_ARGUMENT, "target out of range");

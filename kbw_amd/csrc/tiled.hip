@@ -235,7 +235,9 @@ int32_t upload_strings(kbwh_state *s, const std::vector<SString> &strings) {
 // greedily into groups whose X/Y qubits fit one tile (qubits 0..2 plus up to
 // TL - 3 more); every group is one read-only sweep of the state.  Diagonal
 // strings join the first group.
-int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out) {
+int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const double *coeffs, double *out,
+                        const ExpWork *work) {
+    static_assert(sizeof(SString) == kExpStringBytes, "string records shared with observables.hip");
     const uint32_t n = s->n;
     if (s->block || n < (uint32_t)TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "grouped exp_value needs a flat state of at least 11 qubits");
     std::vector<SGroup> groups;
@@ -247,22 +249,39 @@ int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const 
         return KBWH_OK;
     }
     if (groups.size() > kResultSlots) return set_error(KBWH_ERR_CAPACITY, "too many string groups");
-    rc = upload_strings(s, strings);
-    if (rc) return rc;
+    const SString *ds = nullptr;
+    double *partial = s->partial, *results = s->result;
     hipError_t e = hipSuccess;
+    if (work) {
+        // the caller's work area: the table goes in with one stream-ordered
+        // copy, and the host copy lives until reduce_from synchronizes
+        if (strings.size() > work->strings_cap) return set_error(KBWH_ERR_CAPACITY, "string table larger than the work area");
+        e = hipMemcpyAsync(work->strings, strings.data(), strings.size() * sizeof(SString), hipMemcpyHostToDevice,
+                           s->stream);
+        ds = (const SString *)work->strings;
+        partial = work->partial;
+        results = work->results;
+    } else {
+        rc = upload_strings(s, strings);
+        if (rc) return rc;
+        ds = (const SString *)s->tile_gates;
+    }
     const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
     const uint32_t grid = std::min<uint32_t>(ntiles, 2048);
-    const SString *ds = (const SString *)s->tile_gates;
     for (size_t gi = 0; gi < groups.size() && e == hipSuccess; ++gi) {
         const SGroup &g = groups[gi];
         k_pauli_tile<<<grid, TT, 0, s->stream>>>((const dv2 *)s->amp, g.ts, ds + g.first, g.count, g.has_x, ntiles,
-                                                  s->partial);
+                                                  partial);
         e = hipGetLastError();
-        if (e == hipSuccess) e = launch_sum_final(s->partial, grid, s->result + gi, s->stream);
+        if (e == hipSuccess) e = launch_sum_final(partial, grid, results + gi, s->stream);
     }
-    if (e != hipSuccess) return hip_status(e, "exp_value sweeps");
+    if (e != hipSuccess) {
+        // the string table's host copy must outlive an enqueued upload
+        (void)hipStreamSynchronize(s->stream);
+        return hip_status(e, "exp_value sweeps");
+    }
     std::vector<double> vals(groups.size());
-    rc = reduce_to_host(s, vals.data(), (uint32_t)groups.size());
+    rc = reduce_from(s, results, vals.data(), (uint32_t)groups.size());
     if (rc) return rc;
     double total = 0.0;
     for (double v : vals) total += v;

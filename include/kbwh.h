@@ -59,8 +59,11 @@ extern "C" {
 /* kbwh_new flags */
 #define KBWH_FLAG_BLOCK 1u  /* Block<_, DenseV2> semantics (block.rs): logical/physical map, LRU swaps */
 #define KBWH_FLAG_QUEUE 2u  /* kbwh_gate queues gates; they are applied (same result bits) before any read of the
-                               state: one launch for up to 12 qubits, tiled passes above.  Not with KBWH_FLAG_BLOCK.
-                               kbwh_device_buffer users call kbwh_synchronize first. */
+                               state: one launch for up to 12 qubits, tiled passes above.  With KBWH_FLAG_BLOCK the
+                               queue is Block's gate queue (block.rs:179-208): the map, LRU stamps and swaps are
+                               decided when a gate is given, gates and swaps run together in exact tiled passes
+                               at the flush, and kbwh_measure_collapse does not flush (block.rs:311-316) unless
+                               its target needs a swap.  kbwh_device_buffer users call kbwh_synchronize first. */
 
 /* kbwh_run modes */
 #define KBWH_RUN_PER_GATE 0u /* one HBM pass per gate: the plain path */

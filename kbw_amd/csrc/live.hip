@@ -13,6 +13,18 @@
 // LDS.  A state narrower than 12 qubits is padded with qubits that no gate
 // touches (their amplitudes stay 0 and are neither loaded nor stored).
 // Larger states are flushed through the tiled planner (also exact).
+//
+// A Block state queues gates as the reference's Block does (block.rs:179-208):
+// the target's map and LRU stamp, and the swap of a global target, are decided
+// when the gate is given; the gate is recorded on physical qubits and the swap
+// as an event before it.  The reference flushes before each swap and at
+// measure_p1 and dump (block.rs:132, 302-320); here a flush happens before any
+// read of the state and runs gates and swaps together in exact tiled passes
+// (swaps folded into passes), the same amplitudes either way: a swap moves
+// amplitudes and never recomputes one.  measure_collapse does not flush, as in
+// block.rs:311-316: it applies the queue up to its last swap (the part the
+// reference has already applied) and leaves the gates after it queued; a
+// target that needs a swap flushes everything first.
 #include <string.h>
 
 #include <vector>
@@ -77,6 +89,17 @@ int32_t queue_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint6
     if (kind < KBWH_X || kind > KBWH_RZ) return set_error(KBWH_ERR_INVALID_ARGUMENT, "unknown gate kind");
     if (target >= s->n) return set_error(KBWH_ERR_INVALID_ARGUMENT, "target out of range");
     if (s->n < 64 && (ctrl_mask >> s->n)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "control out of range");
+    if (s->block) {
+        // block.rs:233-300: the target is made local now (the LRU stamp and,
+        // for a global target, the swap and the new map), then the controls
+        // are read through the updated map; the amplitudes move at the flush
+        uint32_t t = target, sl = kNoSwap, sg = kNoSwap;
+        map_qubit(s, target, &t, &sl, &sg);
+        if (sl != kNoSwap) s->block_queue.push_back(kbwh_phys_event{KX, 0, 0, 0.0, sl, sg});
+        const uint64_t m = physical_mask(s, ctrl_mask) & ~(1ull << t);
+        s->block_queue.push_back(kbwh_phys_event{kind, t, m, angle, kNoSwap, kNoSwap});
+        return KBWH_OK;
+    }
     kbwh_op op;
     memset(&op, 0, sizeof op);
     op.kind = (uint8_t)kind;
@@ -88,17 +111,43 @@ int32_t queue_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint6
     return KBWH_OK;
 }
 
+bool queue_pending(const kbwh_state *s) { return !s->queue.empty() || !s->block_queue.empty(); }
+
+// A Block state's queue up to and including its last swap.  The reference
+// flushes before every swap (block.rs:132), so at any moment it holds only the
+// gates given after the last swap; an operation that does not flush
+// (measure_collapse, block.rs:311-316) must see the state with everything
+// before that point applied.
+int32_t flush_through_last_swap(kbwh_state *s) {
+    std::vector<kbwh_phys_event> &q = s->block_queue;
+    size_t last = q.size();
+    for (size_t i = q.size(); i-- > 0;)
+        if (q[i].swap_local != kNoSwap) {
+            last = i;
+            break;
+        }
+    if (last == q.size()) return KBWH_OK;
+    std::vector<kbwh_phys_event> head(q.begin(), q.begin() + (ptrdiff_t)last + 1);
+    q.erase(q.begin(), q.begin() + (ptrdiff_t)last + 1);
+    return run_events(s, s->amp, head);
+}
+
 int32_t flush_queue(kbwh_state *s) {
+    if (!s->block_queue.empty()) {
+        // a Block state's queue (block.rs:179-208, flushed where the reference
+        // flushes or earlier): gates and swaps in program order, in exact
+        // tiled passes with the swaps folded in
+        std::vector<kbwh_phys_event> ev;
+        ev.swap(s->block_queue);
+        uint64_t gates = 0;
+        for (const kbwh_phys_event &E : ev) gates += E.swap_local == kNoSwap;
+        const int32_t rc = run_events(s, s->amp, ev);
+        s->last_gates = gates;
+        return rc;
+    }
     if (s->queue.empty()) return KBWH_OK;
     std::vector<kbwh_op> q;
     q.swap(s->queue);
-    if (s->block) {
-        for (const kbwh_op &op : q) {
-            int32_t rc = apply_gate(s, op.kind, op.angle, op.target, op.ctrl_mask);
-            if (rc) return rc;
-        }
-        return KBWH_OK;
-    }
     if (s->n > kLiveQubits) return run_tiled(s, q.data(), q.size(), nullptr);
     // one pass over a 12-qubit tile holding qubits 0..11 (n..11 are padding)
     Pass all;

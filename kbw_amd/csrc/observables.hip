@@ -205,7 +205,7 @@ int32_t kbwh_exp_value_noalloc(kbwh_state *s, const uint8_t *paulis, const uint3
     if (rc) return rc;
     if (ns > kNoAllocStrings) return set_error(KBWH_ERR_CAPACITY, "kbwh_exp_value_noalloc takes at most 4096 strings");
     // flushing queued gates may grow the gate buffer
-    if (!s->queue.empty())
+    if (queue_pending(s))
         return set_error(KBWH_ERR_INVALID_ARGUMENT, "queued gates pending: call kbwh_synchronize before kbwh_exp_value_noalloc");
     double *base = nullptr;
     if ((rc = scratch_on(s->device, &base))) return rc;

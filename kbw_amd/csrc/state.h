@@ -11,6 +11,18 @@
 
 constexpr uint32_t kResultSlots = 4096;
 
+// One physical operation of a Block state's gate queue, recorded when the gate
+// is given (the map and LRU decisions are made then, as in block.rs:123-156):
+// a gate on physical qubits, or (swap_local != 0xFFFFFFFF) the exchange of
+// physical qubits swap_local and swap_global that brings a global target down.
+struct kbwh_phys_event {
+    int32_t kind;
+    uint32_t target;
+    uint64_t ctrl;  // physical control mask without the target bit
+    double angle;
+    uint32_t swap_local, swap_global;
+};
+
 struct kbwh_state {
     uint32_t n = 0;
     bool block = false;
@@ -42,7 +54,8 @@ struct kbwh_state {
     uint32_t auto_mode = KBWH_RUN_TILED;
     // live gates queued with KBWH_FLAG_QUEUE, applied before the state is read
     bool use_queue = false;
-    std::vector<kbwh_op> queue;
+    std::vector<kbwh_op> queue;               // flat states
+    std::vector<kbwh_phys_event> block_queue;  // Block states
     uint64_t last_gates = 0;     // gates of the last kbwh_run
     // adjoint differentiation: two extra state vectors, allocated on first use
     double2 *adj_phi = nullptr;
@@ -88,6 +101,12 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
                               double *slots, double *partial, size_t partial_cap);
 int32_t queue_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint64_t ctrl_mask);
 int32_t flush_queue(kbwh_state *s);
+bool queue_pending(const kbwh_state *s);
+int32_t flush_through_last_swap(kbwh_state *s);
+// Physical events in program order (gates and Block swaps) in exact tiled
+// passes, each swap folded into a pass when its qubits fit the tile (else a
+// k_swap_bits pass of its own); one launch per op below 11 qubits.
+int32_t run_events(kbwh_state *s, double2 *amp, const std::vector<kbwh_phys_event> &ev);
 int32_t run_fused(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops, const double *params,
                   int k);
 

@@ -191,7 +191,6 @@ int32_t kbwh_new(uint32_t num_qubits, uint32_t flags, uint32_t block_size, kbwh_
     if (!out) return fail(KBWH_ERR_INVALID_ARGUMENT, "out is null");
     *out = nullptr;
     const bool block = (flags & KBWH_FLAG_BLOCK) != 0;
-    if (block && (flags & KBWH_FLAG_QUEUE)) return fail(KBWH_ERR_INVALID_ARGUMENT, "the gate queue is for states without blocks");
     uint32_t nlocal = num_qubits;
     if (block) {
         if (num_qubits <= 1) return fail(KBWH_ERR_EXECUTION_FAILED, "Block needs at least 2 qubits");
@@ -251,6 +250,7 @@ int32_t kbwh_free(kbwh_state *s) {
 int32_t kbwh_clear(kbwh_state *s) {
     if (!s) return fail(KBWH_ERR_INVALID_ARGUMENT, "null handle");
     s->queue.clear();
+    s->block_queue.clear();
     HIPCHK(launch_init(s->amp, s->size, s->stream));
     for (uint32_t i = 0; i < s->n; ++i) s->l2p[i] = s->p2l[i] = i;
     std::fill(s->last_access.begin(), s->last_access.end(), 0);
@@ -308,6 +308,26 @@ int32_t kbwh_measure_p1(kbwh_state *s, uint32_t target, double *p1) {
 
 int32_t kbwh_measure_collapse(kbwh_state *s, uint32_t target, int32_t result, double p) {
     if (!s) return fail(KBWH_ERR_INVALID_ARGUMENT, "null handle");
+    if (s->block && s->use_queue) {
+        // block.rs:311-316: get_local_qubit (a global target flushes the
+        // queue and swaps), then every block collapses; the gates given after
+        // the last swap stay queued and are applied after the collapse, as in
+        // the reference
+        if (target >= s->n) return fail(KBWH_ERR_INVALID_ARGUMENT, "target out of range");
+        uint32_t t = 0, sl = kNoSwap, sg = kNoSwap;
+        map_qubit(s, target, &t, &sl, &sg);
+        int32_t rc;
+        if (sl != kNoSwap) {
+            s->block_queue.push_back(kbwh_phys_event{KBWH_X, 0, 0, 0.0, sl, sg});
+            rc = flush_queue(s);
+        } else {
+            // the reference's queue holds only the gates after its last swap
+            rc = flush_through_last_swap(s);
+        }
+        if (rc) return rc;
+        HIPCHK(launch_collapse(s->amp, s->n, t, result ? 1 : 0, p, s->stream));
+        return KBWH_OK;
+    }
     FLUSH(s);
     return measure_collapse(s, target, result, p);
 }

@@ -345,54 +345,40 @@ extern "C" int32_t kbwh_reorder_plan(const kbwh_op *ops, uint64_t num_ops, uint3
 
 namespace kbwh {
 
-int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params) {
-    return run_tiled_on(s, s->amp, ops, num_ops, params, false);
-}
+namespace {
 
-int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
-                     const double *params, bool reorder) {
-    if (s->n < TL) {
-        // the whole state is smaller than one tile: one pass per gate
-        if (amp == s->amp) return kbwh_run(s, ops, num_ops, params, KBWH_RUN_PER_GATE);
-        for (uint64_t i = 0; i < num_ops; ++i) {
-            const double angle = (params && ops[i].param >= 0) ? params[ops[i].param] : ops[i].angle;
-            hipError_t e = launch_gate(amp, s->n, ops[i].kind, angle, ops[i].target,
-                                       ops[i].ctrl_mask & ~(1ull << ops[i].target), s->stream);
-            if (e != hipSuccess) return hip_status(e, "gate launch");
-        }
-        return KBWH_OK;
-    }
-    if (s->block && amp != s->amp) return set_error(KBWH_ERR_INVALID_ARGUMENT, "Block state on a side buffer");
-    const uint32_t n = s->n;
-    s->last_passes = 0;
-    for (uint64_t i = 0; i < num_ops; ++i) {
-        const kbwh_op &op = ops[i];
-        if (op.kind > KRZ) return set_error(KBWH_ERR_INVALID_ARGUMENT, "unknown gate kind");
-        if (op.target >= n) return set_error(KBWH_ERR_INVALID_ARGUMENT, "target out of range");
-        if (n < 64 && (op.ctrl_mask >> n)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "control out of range");
-    }
-    Pass cur;
-    // Block swaps are folded into the passes.  A pass is planned in the
-    // qubits of the state as its tiles are loaded; ld[p] is the load-time
-    // qubit whose amplitudes now sit at physical qubit p (after the swaps
-    // made so far in the pass) and ph is its inverse.  Each swap must keep
-    // both of its qubits in the tile; the pass stores the tile through the
-    // permuted map.
-    std::vector<uint32_t> ld(n), ph(n);
-    for (uint32_t q = 0; q < n; ++q) ld[q] = ph[q] = q;
-    bool swapped = false;
-    std::vector<TGate> gates;
+// Builds tiled passes from physical ops and Block swaps and launches them.
+// A pass is planned in the qubits of the state as its tiles are loaded; ld[p]
+// is the load-time qubit whose amplitudes now sit at physical qubit p (after
+// the swaps folded into the pass so far) and ph is its inverse.  Each swap
+// keeps both of its qubits in the tile; the pass stores the tile through the
+// permuted map.
+struct PassBuilder {
+    static_assert(sizeof(TileSpec) == kTileSpecBytes && sizeof(TGate) == kTGateBytes,
+                  "tile records shared with tiled_fma.hip");
     struct Launch {
         TileSpec spec, store;
         uint32_t first, count;
         bool perm;
     };
-    static_assert(sizeof(TileSpec) == kTileSpecBytes && sizeof(TGate) == kTGateBytes,
-                  "tile records shared with tiled_fma.hip");
-    const bool contract = reorder && !s->block;
+    kbwh_state *s;
+    double2 *amp;
+    uint32_t n, cap;
+    bool contract;  // flat dependency-ordered runs: the contracted kernel copy
+    Pass cur;
+    std::vector<uint32_t> ld, ph;
+    bool swapped = false;
+    std::vector<TGate> gates;
     std::vector<Launch> launches;
-    const uint32_t cap = std::min<uint32_t>(s->tile_targets, TL - 3);
-    auto flush = [&]() -> int32_t {
+
+    PassBuilder(kbwh_state *st, double2 *a, bool contracted)
+        : s(st), amp(a), n(st->n), cap(std::min<uint32_t>(st->tile_targets, TL - 3)), contract(contracted),
+          ld(st->n), ph(st->n) {
+        for (uint32_t q = 0; q < n; ++q) ld[q] = ph[q] = q;
+    }
+
+    // closes the pending pass into a launch record
+    int32_t flush() {
         if (cur.ops.empty() && !swapped) return KBWH_OK;
         int pos_of[64];
         uint32_t q_at[TL];
@@ -429,8 +415,10 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         for (uint32_t q = 0; q < n; ++q) ld[q] = ph[q] = q;
         swapped = false;
         return KBWH_OK;
-    };
-    auto launch_all = [&]() -> int32_t {
+    }
+
+    // uploads the gate records and launches every closed pass
+    int32_t launch_all() {
         if (launches.empty()) return KBWH_OK;
         const size_t bytes = gates.size() * sizeof(TGate);
         if (s->tile_gates_cap < bytes) {
@@ -460,80 +448,180 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         launches.clear();
         gates.clear();
         return KBWH_OK;
-    };
+    }
+
     // tile positions a swap of physical qubits a and b still needs in the pending pass
-    auto swap_needs = [&](uint32_t a, uint32_t b) -> uint32_t {
+    uint32_t swap_needs(uint32_t a, uint32_t b) const {
         uint32_t need = 0;
         for (uint32_t q : {ld[a], ld[b]})
             if (q >= 3 && std::find(cur.targets.begin(), cur.targets.end(), q) == cur.targets.end()) ++need;
         return need;
-    };
-    auto require = [&](uint32_t q) {
+    }
+    void require(uint32_t q) {
         if (q >= 3 && std::find(cur.targets.begin(), cur.targets.end(), q) == cur.targets.end())
             cur.targets.push_back(q);
-    };
-    if (reorder && s->block && cap >= 2) {
-        // A Block state: the LRU map and its swaps follow program order (the
-        // map is replayed here first, exactly as gate by gate), then swaps and
-        // gates are scheduled as ops on physical qubits.  A swap needs both of
-        // its qubits in the tile and is folded into its pass as above.  The
-        // ops touching each physical qubit keep their order, so the relabelling
-        // of every qubit is the same as in program order.
-        struct Ev {
-            PhysOp op;
-            uint32_t sl, sg;  // sl != kNoSwap: an exchange of physical qubits sl and sg
-        };
-        std::vector<Ev> ev;
+    }
+    // folds the exchange of physical qubits a and b into the pending pass
+    void fold_swap(uint32_t a, uint32_t b) {
+        require(ld[a]);
+        require(ld[b]);
+        std::swap(ld[a], ld[b]);
+        ph[ld[a]] = a;
+        ph[ld[b]] = b;
+        swapped = true;
+    }
+    // a gate on physical qubits, written against the load-time qubits
+    PhysOp load_time(const PhysOp &o) const {
+        uint64_t lm = 0;
+        for (uint64_t r = o.ctrl; r; r &= r - 1) lm |= 1ull << ld[__builtin_ctzll(r)];
+        PhysOp p = o;
+        p.target = ld[o.target];
+        p.ctrl = lm;
+        return p;
+    }
+
+    // program order: an op joins the pending pass while its target fits
+    int32_t program_order(const std::vector<kbwh_phys_event> &ev) {
+        for (const kbwh_phys_event &E : ev) {
+            if (E.swap_local != kNoSwap) {
+                // a global target exchanged with a local qubit (block.rs:132-150)
+                const uint32_t sl = E.swap_local, sg = E.swap_global;
+                int32_t rc = KBWH_OK;
+                if (cur.targets.size() + swap_needs(sl, sg) > cap) rc = flush();
+                if (rc) return rc;
+                if (swap_needs(sl, sg) > cap) {
+                    // cannot share a tile under this target cap: a swap pass of its own
+                    rc = launch_all();
+                    if (rc) return rc;
+                    hipError_t e = launch_swap_bits(amp, n, sl, sg, s->stream);
+                    if (e != hipSuccess) return hip_status(e, "swap launch");
+                    ++s->last_passes;
+                } else {
+                    fold_swap(sl, sg);
+                }
+                continue;
+            }
+            const PhysOp op{(int)E.kind, E.target, E.ctrl, E.angle};
+            PhysOp po = load_time(op);
+            if (!fits(cur, po, cap)) {
+                int32_t rc = flush();
+                if (rc) return rc;
+                po = load_time(op);
+            }
+            add_op(cur, po);
+        }
+        int32_t rc = flush();
+        if (rc == KBWH_OK) rc = launch_all();
+        return rc;
+    }
+};
+
+int32_t check_ops(uint32_t n, const kbwh_op *ops, uint64_t num_ops) {
+    for (uint64_t i = 0; i < num_ops; ++i) {
+        const kbwh_op &op = ops[i];
+        if (op.kind > KRZ) return set_error(KBWH_ERR_INVALID_ARGUMENT, "unknown gate kind");
+        if (op.target >= n) return set_error(KBWH_ERR_INVALID_ARGUMENT, "target out of range");
+        if (n < 64 && (op.ctrl_mask >> n)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "control out of range");
+    }
+    return KBWH_OK;
+}
+
+}  // namespace
+
+int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params) {
+    return run_tiled_on(s, s->amp, ops, num_ops, params, false);
+}
+
+int32_t run_events(kbwh_state *s, double2 *amp, const std::vector<kbwh_phys_event> &ev) {
+    s->last_passes = 0;
+    if (s->n < TL) {
+        // the whole state is smaller than one tile: one launch per op
+        for (const kbwh_phys_event &E : ev) {
+            const hipError_t e = E.swap_local != kNoSwap
+                                     ? launch_swap_bits(amp, s->n, E.swap_local, E.swap_global, s->stream)
+                                     : launch_gate(amp, s->n, E.kind, E.angle, E.target, E.ctrl, s->stream);
+            if (e != hipSuccess) return hip_status(e, "gate launch");
+            ++s->last_passes;
+        }
+        return KBWH_OK;
+    }
+    PassBuilder pb(s, amp, false);
+    return pb.program_order(ev);
+}
+
+int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
+                     const double *params, bool reorder) {
+    if (s->n < TL) {
+        // the whole state is smaller than one tile: one pass per gate
+        if (amp == s->amp) return kbwh_run(s, ops, num_ops, params, KBWH_RUN_PER_GATE);
+        for (uint64_t i = 0; i < num_ops; ++i) {
+            const double angle = (params && ops[i].param >= 0) ? params[ops[i].param] : ops[i].angle;
+            hipError_t e = launch_gate(amp, s->n, ops[i].kind, angle, ops[i].target,
+                                       ops[i].ctrl_mask & ~(1ull << ops[i].target), s->stream);
+            if (e != hipSuccess) return hip_status(e, "gate launch");
+        }
+        return KBWH_OK;
+    }
+    if (s->block && amp != s->amp) return set_error(KBWH_ERR_INVALID_ARGUMENT, "Block state on a side buffer");
+    const uint32_t n = s->n;
+    s->last_passes = 0;
+    int32_t rc = check_ops(n, ops, num_ops);
+    if (rc) return rc;
+    PassBuilder pb(s, amp, reorder && !s->block);
+    // the map replayed in program order, exactly as gate by gate: a Block
+    // state's swaps become events before the gates that need them
+    auto to_events = [&](std::vector<kbwh_phys_event> &ev) {
         ev.reserve(num_ops + num_ops / 4);
         for (uint64_t i = 0; i < num_ops; ++i) {
             const kbwh_op &op = ops[i];
             const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
             uint32_t t = op.target, sl = kNoSwap, sg = kNoSwap;
-            map_qubit(s, op.target, &t, &sl, &sg);
-            if (sl != kNoSwap) ev.push_back(Ev{PhysOp{KX, 0, 0, 0.0}, sl, sg});
+            if (s->block) map_qubit(s, op.target, &t, &sl, &sg);
+            if (sl != kNoSwap) ev.push_back(kbwh_phys_event{KX, 0, 0, 0.0, sl, sg});
             const uint64_t m = physical_mask(s, op.ctrl_mask) & ~(1ull << t);
-            ev.push_back(Ev{PhysOp{(int)op.kind, t, m, angle}, kNoSwap, kNoSwap});
+            ev.push_back(kbwh_phys_event{(int32_t)op.kind, t, m, angle, kNoSwap, kNoSwap});
         }
+    };
+    if (reorder && s->block && pb.cap >= 2) {
+        // A Block state: the LRU map and its swaps follow program order (the
+        // map is replayed first, exactly as gate by gate), then swaps and
+        // gates are scheduled as ops on physical qubits.  A swap needs both of
+        // its qubits in the tile and is folded into its pass.  The ops
+        // touching each physical qubit keep their order, so the relabelling
+        // of every qubit is the same as in program order.
+        std::vector<kbwh_phys_event> ev;
+        to_events(ev);
         std::vector<uint64_t> qm(ev.size()), req(ev.size());
         for (size_t i = 0; i < ev.size(); ++i) {
-            const Ev &E = ev[i];
-            if (E.sl != kNoSwap) {
-                qm[i] = req[i] = (1ull << E.sl) | (1ull << E.sg);
+            const kbwh_phys_event &E = ev[i];
+            if (E.swap_local != kNoSwap) {
+                qm[i] = req[i] = (1ull << E.swap_local) | (1ull << E.swap_global);
             } else {
-                qm[i] = E.op.ctrl | (1ull << E.op.target);
-                req[i] = is_diagonal(E.op.kind) ? 0ull : (1ull << E.op.target);
+                qm[i] = E.ctrl | (1ull << E.target);
+                req[i] = is_diagonal(E.kind) ? 0ull : (1ull << E.target);
             }
         }
         std::vector<uint32_t> order, ends;
-        schedule_ops(qm, req, n, cap, order, ends);
+        schedule_ops(qm, req, n, pb.cap, order, ends);
         if (order.size() != ev.size()) return set_error(KBWH_ERR_DEVICE, "Block schedule incomplete");
         size_t k = 0;
         for (const uint32_t end : ends) {
             for (; k < end; ++k) {
-                const Ev &E = ev[order[k]];
-                if (E.sl != kNoSwap) {
-                    require(ld[E.sl]);
-                    require(ld[E.sg]);
-                    std::swap(ld[E.sl], ld[E.sg]);
-                    ph[ld[E.sl]] = E.sl;
-                    ph[ld[E.sg]] = E.sg;
-                    swapped = true;
-                    continue;
-                }
-                uint64_t lm = 0;
-                for (uint64_t r = E.op.ctrl; r; r &= r - 1) lm |= 1ull << ld[__builtin_ctzll(r)];
-                add_op(cur, PhysOp{E.op.kind, ld[E.op.target], lm, E.op.angle});
+                const kbwh_phys_event &E = ev[order[k]];
+                if (E.swap_local != kNoSwap)
+                    pb.fold_swap(E.swap_local, E.swap_global);
+                else
+                    add_op(pb.cur, pb.load_time(PhysOp{(int)E.kind, E.target, E.ctrl, E.angle}));
             }
-            const int32_t rc = flush();
-            if (rc) return rc;
+            if ((rc = pb.flush())) return rc;
         }
-        const int32_t rc = launch_all();
+        rc = pb.launch_all();
         s->last_gates = num_ops;
         return rc;
     }
     if (reorder && !s->block) {
-        // dependency-ordered passes (a Block state keeps program order: its
-        // swaps follow the order in which targets are met)
+        // dependency-ordered passes (a Block state under a target cap of one
+        // keeps program order: its swaps cannot share a tile)
         std::vector<PhysOp> phys(num_ops);
         for (uint64_t i = 0; i < num_ops; ++i) {
             const kbwh_op &op = ops[i];
@@ -542,58 +630,19 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         }
         fuse_diagonals(phys, n);
         std::vector<uint32_t> order, ends;
-        schedule_passes(phys, n, cap, order, ends);
+        schedule_passes(phys, n, pb.cap, order, ends);
         size_t k = 0;
         for (const uint32_t end : ends) {
-            for (; k < end; ++k) add_op(cur, phys[order[k]]);
-            const int32_t rc = flush();
-            if (rc) return rc;
+            for (; k < end; ++k) add_op(pb.cur, phys[order[k]]);
+            if ((rc = pb.flush())) return rc;
         }
-        const int32_t rc = launch_all();
+        rc = pb.launch_all();
         s->last_gates = num_ops;
         return rc;
     }
-    for (uint64_t i = 0; i < num_ops; ++i) {
-        const kbwh_op &op = ops[i];
-        const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
-        uint32_t t = op.target, sl = kNoSwap, sg = kNoSwap;
-        if (s->block) map_qubit(s, op.target, &t, &sl, &sg);
-        if (sl != kNoSwap) {
-            // a global target exchanged with a local qubit (block.rs:132-150)
-            int32_t rc = KBWH_OK;
-            if (cur.targets.size() + swap_needs(sl, sg) > cap) rc = flush();
-            if (rc) return rc;
-            if (swap_needs(sl, sg) > cap) {
-                // cannot share a tile under this target cap: a swap pass of its own
-                rc = launch_all();
-                if (rc) return rc;
-                hipError_t e = launch_swap_bits(amp, n, sl, sg, s->stream);
-                if (e != hipSuccess) return hip_status(e, "swap launch");
-            } else {
-                require(ld[sl]);
-                require(ld[sg]);
-                std::swap(ld[sl], ld[sg]);
-                ph[ld[sl]] = sl;
-                ph[ld[sg]] = sg;
-                swapped = true;
-            }
-        }
-        const uint64_t m = physical_mask(s, op.ctrl_mask) & ~(1ull << t);
-        auto load_time = [&]() {
-            uint64_t lm = 0;
-            for (uint64_t r = m; r; r &= r - 1) lm |= 1ull << ld[__builtin_ctzll(r)];
-            return PhysOp{(int)op.kind, ld[t], lm, angle};
-        };
-        PhysOp po = load_time();
-        if (!fits(cur, po, cap)) {
-            int32_t rc = flush();
-            if (rc) return rc;
-            po = load_time();
-        }
-        add_op(cur, po);
-    }
-    int32_t rc = flush();
-    if (rc == KBWH_OK) rc = launch_all();
+    std::vector<kbwh_phys_event> ev;
+    to_events(ev);
+    rc = pb.program_order(ev);
     s->last_gates = num_ops;
     return rc;
 }

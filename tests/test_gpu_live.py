@@ -1,6 +1,8 @@
 """Queued live gates (KBWH_FLAG_QUEUE): gates given one by one are applied in
 one launch (up to 12 qubits) or in tiled passes before the state is read, with
 the same result bits as gate by gate, against the oracle."""
+import math
+
 import numpy as np
 import pytest
 
@@ -91,5 +93,109 @@ def test_clear_discards_the_queue_and_errors_are_immediate():
     assert st[0] == 1 and np.count_nonzero(st) == 1
     with pytest.raises(KbwhError):
         q.hadamard(6)  # target out of range, reported when given
-    with pytest.raises(KbwhError):
-        DenseHip(8, block=True, block_size=6, queue=True)
+    # a Block state's queue and map go back to the start too
+    b = DenseHip(8, block=True, block_size=5, queue=True)
+    b.hadamard(7)
+    b.pauli_x(6, [7])
+    assert b.swap_count() == 2 and list(b.qubit_map()) != list(range(8))
+    b.clear()
+    st = b.state()
+    assert st[0] == 1 and np.count_nonzero(st) == 1
+    assert list(b.qubit_map()) == list(range(8))
+
+
+def block_live(o, ops, n):
+    for op in ops:
+        o.gate(int(op["kind"]), float(op["angle"]), int(op["target"]), ctrl_list(int(op["ctrl_mask"]), n))
+
+
+@pytest.mark.parametrize("n,L", [(6, 3), (9, 5), (12, 8), (16, 11), (20, 14)])
+def test_queued_block_gates_bit_exact(n, L):
+    """Block's gate queue (block.rs:179-208): the map, LRU stamps and swap
+    count follow each gate as it is given (compared before anything is
+    flushed); at the flush, gates and swaps run in exact tiled passes, and the
+    physical state, dump and samples equal the oracle Block's bit for bit."""
+    print(f"queued block n={n} L={L}", flush=True)
+    rng = np.random.default_rng(500 + n)
+    ops = circuits.random_circuit(rng, n, 240, max_controls=3)
+    q = DenseHip(n, block=True, block_size=L, queue=True)
+    live(q, ops, n)
+    o = oracle.Block(n, L)
+    block_live(o, ops, n)
+    assert np.array_equal(q.qubit_map(), o.qubit_map())
+    assert q.swap_count() == o.swap_count() > 0
+    st = q.state()  # flushes
+    assert np.array_equal(st, o.physical_state())
+    gates, passes = q.run_stats()
+    assert gates == len(ops)
+    if n >= 11:
+        assert passes < len(ops) // 4, passes  # many gates and swaps per pass
+    qubits = list(range(n))
+    d = q.dump(qubits)
+    k, re, im = o.dump(qubits)
+    assert np.array_equal(d.keys, k) and np.array_equal(d.amplitudes_real, re)
+    assert np.array_equal(d.amplitudes_imag, im)
+    live(q, ops[:60], n)
+    block_live(o, ops[:60], n)
+    u = rng.random(4000)
+    phys = [int(x) for x in o.qubit_map()]
+    samp = q.sample(qubits, len(u), u)  # flushes the 60 new gates first
+    assert np.array_equal(samp, oracle.sample_hier(o.physical_state(), n, phys, len(u), u))
+
+
+def oracle_block_measure(o, qubits, uniforms):
+    """QubitManager::do_measure (quantum_execution.rs:272-292) over the oracle
+    Block: reverse order, p0 = max(1 - p1, 0), rand's WeightedIndex pick."""
+    out = 0
+    for idx, q in enumerate(reversed(qubits)):
+        p1 = o.measure_p1(q)
+        p0 = max(1.0 - p1, 0.0)
+        x = uniforms[idx] * oracle.uniform_scale(p0 + p1) + 0.0
+        r = 1 if p0 <= x else 0
+        prob = p1 if r else p0
+        o.measure_collapse(q, bool(r), 1.0 / math.sqrt(max(prob, 2.2250738585072014e-308)))
+        out |= r << idx
+    return out
+
+
+def test_queued_block_measure_loop():
+    n, L = 14, 10
+    rng = np.random.default_rng(88)
+    ops = circuits.random_circuit(rng, n, 200, max_controls=2)
+    q = DenseHip(n, block=True, block_size=L, queue=True)
+    o = oracle.Block(n, L)
+    for rnd in range(3):
+        live(q, ops[rnd * 60:(rnd + 1) * 60], n)
+        block_live(o, ops[rnd * 60:(rnd + 1) * 60], n)
+        m = [int(x) for x in rng.permutation(n)[:5]]
+        u = rng.random(5)
+        assert q.measure(m, u) == oracle_block_measure(o, m, u)
+        assert np.array_equal(q.qubit_map(), o.qubit_map())
+        assert q.swap_count() == o.swap_count()
+        # the collapse factors come from each side's own p1 (equal to a few
+        # ulps), so the states agree to rounding after a measurement
+        assert np.max(np.abs(q.state() - o.physical_state())) < 1e-13
+
+
+def test_queued_block_collapse_before_queued_gates():
+    """block.rs:311-316: measure_collapse does not flush the queue, so gates
+    given before it are applied after it (the reference's order); a global
+    target first flushes and swaps.  Same factor p on both sides: bit-exact."""
+    n, L = 12, 8
+    rng = np.random.default_rng(91)
+    ops = circuits.random_circuit(rng, n, 120, max_controls=2)
+    q = DenseHip(n, block=True, block_size=L, queue=True)
+    o = oracle.Block(n, L)
+    live(q, ops, n)
+    block_live(o, ops, n)
+    local_q = next(l for l in range(n) if int(o.qubit_map()[l]) < L)
+    global_q = next(l for l in range(n) if int(o.qubit_map()[l]) >= L)
+    for tq, res in ((local_q, True), (global_q, False)):
+        q.measure_collapse(tq, res, 1.75)
+        o.measure_collapse(tq, res, 1.75)
+        more = circuits.random_circuit(rng, n, 30, max_controls=2)
+        live(q, more, n)
+        block_live(o, more, n)
+    assert np.array_equal(q.qubit_map(), o.qubit_map())
+    assert q.swap_count() == o.swap_count()
+    assert np.array_equal(q.state(), o.physical_state())

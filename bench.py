@@ -29,6 +29,7 @@ circuit in FP64 MFMA blocks of 4 and 5 qubits).
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -152,6 +153,7 @@ def adjoint_leg(subset=4, batch=256):
     for _ in range(reps):
         e, g = sim.adjoint_gradient(ops, params, ham)
     t_adj = (_t.perf_counter() - t0) / reps
+    adj_roof = adjoint_roofline(sim.adjoint_stats(), 1 << n, 1, t_adj)
     idx = list(np.linspace(0, len(params) - 1, subset).astype(int))
     kag.energy(sim, ops, params, ham, RUN_PER_GATE)
     t0 = _t.perf_counter()
@@ -182,7 +184,8 @@ def adjoint_leg(subset=4, batch=256):
            "max_abs_diff_vs_shift": err, "shift_subset": [int(i) for i in idx],
            "tolerance": 1e-9, "within_tolerance": err <= 1e-9,
            "max_abs_diff_vs_central_fd": err_fd, "fd_step": h,
-           "fd_rounding_bound": float(np.finfo(float).eps * max(1.0, abs(e)) / h)}
+           "fd_rounding_bound": float(np.finfo(float).eps * max(1.0, abs(e)) / h),
+           "roofline": adj_roof}
     if batch:
         nb = 16
         bops, _ = circuits.hea(nb, 12, seed=9)
@@ -190,15 +193,46 @@ def adjoint_leg(subset=4, batch=256):
         rng = np.random.default_rng(11)
         sets = rng.uniform(-np.pi, np.pi, size=(batch, 12 * 2 * nb))
         bs = DenseHip(nb)
+        # all sets in one sweep (kbwh_adjoint_batch)
+        bs.adjoint_batch(bops, sets, bham)
+        reps_b = 5
+        t0 = _t.perf_counter()
+        for _ in range(reps_b):
+            eb, gb = bs.adjoint_batch(bops, sets, bham)
+        t_b = (_t.perf_counter() - t0) / reps_b
+        roof_b = adjoint_roofline(bs.adjoint_stats(), 1 << nb, batch, t_b)
+        # one kbwh_adjoint_gradient call per set, on a sample of the sets
+        loop = min(batch, 64)
         bs.adjoint_gradient(bops, sets[0], bham)
         t0 = _t.perf_counter()
-        for p in sets:
-            bs.adjoint_gradient(bops, p, bham)
-        t_b = _t.perf_counter() - t0
+        diff = 0.0
+        for k in range(loop):
+            e1, g1 = bs.adjoint_gradient(bops, sets[k], bham)
+            diff = max(diff, float(np.max(np.abs(g1 - gb[k]))), abs(e1 - eb[k]))
+        t_l = (_t.perf_counter() - t0) / loop
         bs.close()
         out["batch16"] = {"sets": batch, "params_per_set": sets.shape[1], "s": t_b,
-                          "grads_per_s": batch * sets.shape[1] / t_b}
+                          "grads_per_s": batch * sets.shape[1] / t_b,
+                          "per_set_calls_grads_per_s": sets.shape[1] / t_l,
+                          "speedup_vs_per_set_calls": t_l * batch / t_b,
+                          "max_abs_diff_vs_per_set_calls": diff, "per_set_calls_sample": loop,
+                          "roofline": roof_b}
     return out
+
+
+def adjoint_roofline(stats, amps, sets, seconds):
+    """Algorithmic HBM bytes of adjoint gradients per second against 8 TB/s:
+    per amplitude of each set, a forward pass reads and writes phi (32 B), a
+    lambda sweep reads phi and reads and writes lambda (48 B), the energy reads
+    both (32 B) and a reverse pass reads and writes both (64 B)."""
+    fwd, lam, rev = stats
+    per_amp = 32 * fwd + 48 * lam + 32 + 64 * rev
+    total = float(per_amp) * amps * sets
+    gbps = total / seconds / 1e9
+    return {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,
+            "traffic": None, "bytes_per_gradient": per_amp * amps,
+            "passes": {"forward": fwd, "lambda_sweeps": lam, "reverse": rev},
+            "note": "whole-gradient time (all kernels and host work) over algorithmic bytes"}
 
 
 def batch_leg(num=4096, reps=3, single_sample=64, cpu_sample=96):
@@ -274,7 +308,8 @@ def batch_leg(num=4096, reps=3, single_sample=64, cpu_sample=96):
                                "cores": threads, "kind": "port",
                                "sample": f"{cpu_sample} seeded circuits of the batch on the oracle "
                                          f"(OpenMP {threads} threads), {dt:.2f} s, scaled to the batch "
-                                         f"by total amplitudes"}
+                                         f"by total amplitudes",
+                               "host": host_info()}
     return res
 
 
@@ -396,8 +431,43 @@ def live_leg(reps=5):
         o.run(ops)
     dt = (time.perf_counter() - t0) / reps
     out["cpu_baseline"] = {"value": len(rows) / dt, "unit": "gates/s", "cores": threads, "kind": "port",
-                           "sample": f"the same {len(rows)} gates on the oracle, {reps} runs"}
+                           "sample": f"the same {len(rows)} gates on the oracle, {reps} runs",
+                           "host": host_info()}
     return out
+
+
+def host_info():
+    """The host the CPU baselines run on: CPU model, sockets, physical and
+    logical cores (lscpu), memory (/proc/meminfo)."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "1") or 1)
+        per = int(kv.get("Core(s) per socket", "0") or 0)
+        info["sockets"] = sockets
+        info["physical_cores"] = sockets * per if per else None
+        info["threads_per_core"] = int(kv.get("Thread(s) per core", "1") or 1)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    info["ram_GiB"] = round(int(line.split()[1]) / 2 ** 20, 1)
+    except OSError:
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return info
 
 
 def cpu_baseline(n, ops, ngates):
@@ -412,7 +482,8 @@ def cpu_baseline(n, ops, ngates):
     del s
     return {"value": len(sub) / dt, "unit": "gates/s", "cores": threads, "kind": "port",
             "sample": f"first {len(sub)} gates of the same {n}-qubit brickwork circuit on the oracle "
-                      f"(C restatement of dense_v2.rs, OpenMP {threads} threads), {dt:.2f} s"}
+                      f"(C restatement of dense_v2.rs, OpenMP {threads} threads), {dt:.2f} s",
+            "host": host_info()}
 
 
 def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12):
@@ -569,7 +640,8 @@ def main():
                               "kernel": "k_tile (11-qubit register tiles, all gates of a pass)",
                               "launch_ms": launch_ms, "bytes_per_launch": per_launch,
                               "launches": passes * args.steps}
-        result["hbm_TBps_algorithmic_per_gate_equivalent"] = alg_bytes * args.steps / el / 1e12
+        result["hbm_TBps"] = per_launch * passes * args.steps / el / 1e12
+        result["one_pass_per_gate_equivalent_TBps"] = alg_bytes * args.steps / el / 1e12
         result["tiled"] = {"gates_per_s": result["value"], "ms_per_step": result["ms_per_step"],
                            "passes_per_step": passes, "exact": "bit-exact with program order",
                            "roofline": result["roofline"]}
@@ -581,7 +653,8 @@ def main():
                            "roofline": ro["roofline"]})
             config.update({"run_mode": "reorder", "passes_per_step": ro["passes_per_step"],
                            "gates_per_pass": nops / ro["passes_per_step"]})
-            result["hbm_TBps_algorithmic_per_gate_equivalent"] = alg_bytes / (ro["ms_per_step"] * 1e-3) / 1e12
+            result["hbm_TBps"] = per_launch * ro["passes_per_step"] / (ro["ms_per_step"] * 1e-3) / 1e12
+            result["one_pass_per_gate_equivalent_TBps"] = alg_bytes / (ro["ms_per_step"] * 1e-3) / 1e12
     elif pg is not None:
         result.update({"value": pg["gates_per_s"], "ms_per_step": pg["ms_per_step"]})
         config["run_mode"] = "per_gate"
@@ -589,6 +662,10 @@ def main():
     result["config"] = config
 
     result["traffic_note"] = "traffic = FETCH_SIZE*1.99992 + WRITE_SIZE*1.0 per launch (scale from a 1 GiB copy)"
+    result["hbm_note"] = ("hbm_TBps: bytes the value path's passes move (one read and one write of the "
+                          "state each) per second; one_pass_per_gate_equivalent_TBps: the bytes the plain "
+                          "path would move for the same gates (one pass per gate) per second, which is not "
+                          "an HBM rate and may exceed the 8 TB/s peak")
     # expectation value of the 64-string Hamiltonian
     if "exp" in legs:
         t1 = time.perf_counter()

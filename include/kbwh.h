@@ -210,6 +210,29 @@ int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
                               const double *coeffs, uint32_t num_strings, double *energy,
                               double *grad);
 
+/* The same gradient for num_sets parameter sets of one circuit: set k's
+ * angles are params[k * num_params + j]; energies[k] and grads[k * num_params
+ * + j] receive its energy and gradient.  From 11 qubits the sets run in one
+ * sweep: laid out one after another they form one state of
+ * n + ceil(log2 num_sets) qubits whose top bits select the set, every tiled
+ * pass covers all sets with each set's own gate records (coefficients
+ * computed on the device, so values agree with kbwh_adjoint_gradient to
+ * rounding), and energies and gradient terms are reduced per set.  Needs
+ * 2^(n + ceil(log2 num_sets) + 1) amplitudes of device memory (phi and lambda
+ * of every set).  Below 11 qubits the sets are run one by one.  At most 65535
+ * sets. */
+int32_t kbwh_adjoint_batch(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                           uint32_t num_params, uint32_t num_sets, const uint8_t *paulis, const uint32_t *qubits,
+                           const uint32_t *offsets, const double *coeffs, uint32_t num_strings, double *energies,
+                           double *grads);
+
+/* HBM passes of the last kbwh_adjoint_gradient / kbwh_adjoint_batch (no
+ * counterpart in the reference): forward passes over phi, sweeps building
+ * lambda = H phi (one read of phi and one read-modify-write of lambda each),
+ * reverse passes (each reads and writes phi and lambda once). */
+int32_t kbwh_adjoint_stats(const kbwh_state *s, uint64_t *forward_passes, uint64_t *lambda_sweeps,
+                           uint64_t *reverse_passes);
+
 /* ---- batches of circuits (no counterpart in the reference: its
  * BatchExecution replays one circuit per call on a fresh simulator,
  * quantum_execution.rs:367-427) --------------------------------------------

@@ -59,6 +59,8 @@ SIGNATURES = {
     "kbwh_exp_value": [_vp, _p, _p, _p, _p, _u32, C.POINTER(_f64)],
     "kbwh_exp_value_noalloc": [_vp, _p, _p, _p, _p, _u32, C.POINTER(_f64)],
     "kbwh_adjoint_gradient": [_vp, _p, _u64, _p, _u32, _p, _p, _p, _p, _u32, C.POINTER(_f64), _p],
+    "kbwh_adjoint_batch": [_vp, _p, _u64, _p, _u32, _u32, _p, _p, _p, _p, _u32, _p, _p],
+    "kbwh_adjoint_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)],
     "kbwh_read_amplitudes": [_vp, _u64, _u64, _p],
     "kbwh_write_amplitudes": [_vp, _u64, _u64, _p],
     "kbwh_qubit_map": [_vp, _p],

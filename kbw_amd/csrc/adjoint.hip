@@ -157,6 +157,36 @@ __global__ __launch_bounds__(kB) void k_adj_param(double2 *__restrict__ phi, dou
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// |0...0> in each of nsets states of 2^n amplitudes laid out one after another
+__global__ __launch_bounds__(kB) void k_init_sets(double2 *__restrict__ a, uint64_t total, uint64_t set_mask) {
+    const uint64_t i = (uint64_t)blockIdx.x * kB + threadIdx.x;
+    if (i >= total) return;
+    a[i] = make_double2((i & set_mask) == 0 ? 1.0 : 0.0, 0.0);
+}
+
+// partial[set * gridDim.x + b] = Re <phi|lam> over chunk b of set blockIdx.y
+__global__ __launch_bounds__(kB) void k_dot_sets(const double2 *__restrict__ phi, const double2 *__restrict__ lam,
+                                                 uint64_t set_size, double *__restrict__ partial) {
+    const uint64_t chunk = set_size / gridDim.x;
+    const uint64_t first = (uint64_t)blockIdx.y * set_size + (uint64_t)blockIdx.x * chunk;
+    double acc = 0.0;
+    for (uint64_t i = threadIdx.x; i < chunk; i += kB) {
+        const double2 x = phi[first + i], y = lam[first + i];
+        acc += x.x * y.x + x.y * y.y;
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[(uint64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// out[set] = the sum of the set's m partials, in order
+__global__ __launch_bounds__(kB) void k_sum_sets(const double *__restrict__ partial, uint32_t m,
+                                                 double *__restrict__ out) {
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < m; i += kB) acc += partial[(uint64_t)blockIdx.x * m + i];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
 inline uint32_t grid_for(uint64_t items, uint64_t per) { return (uint32_t)((items + per - 1) / per); }
 
 // phi and lambda in one allocation, lambda right after phi (the tiled
@@ -177,6 +207,10 @@ int32_t ensure_vecs(kbwh_state *s) {
 void free_adjoint(kbwh_state *s) {
     if (s->adj_phi) (void)hipFree(s->adj_phi);  // lambda shares the allocation
     if (s->adj_partial) (void)hipFree(s->adj_partial);
+    kbwh_set_buffers &sb = s->sets;
+    for (void *p : {sb.vecs, sb.gates, sb.tmpl, sb.params, sb.partial})
+        if (p) (void)hipFree(p);
+    sb = kbwh_set_buffers();
     s->adj_phi = s->adj_lam = nullptr;
     s->adj_partial = nullptr;
     s->adj_partial_cap = 0;
@@ -243,6 +277,9 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
     if (e != hipSuccess) return hip_status(e, "adjoint init");
     rc = run_tiled_on(s, s->adj_phi, ops, num_ops, params, true);  // dependency-ordered passes
     if (rc) return rc;
+    s->adj_forward = n >= 11 ? s->last_passes : num_ops;
+    s->adj_lambda = 0;
+    s->adj_reverse = n >= 11 ? 0 : nparam_ops;
     // lambda = H phi and E = Re <phi|lambda>
     const uint32_t g1 = grid_for(size, kB);
     if (num_strings == 0) {
@@ -268,10 +305,13 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
     }
     bool assigned = false;
     if (!narrow.empty()) {
-        rc = pauli_apply_tiled(s, s->adj_phi, s->adj_lam, narrow, narrow_coeffs.data());
+        uint32_t sweeps = 0;
+        rc = pauli_apply_tiled(s, s->adj_phi, s->adj_lam, narrow, narrow_coeffs.data(), 0, 0, &sweeps);
         if (rc) return rc;
+        s->adj_lambda += sweeps;
         assigned = true;
     }
+    s->adj_lambda += wide.size();
     for (size_t w = 0; w < wide.size() && e == hipSuccess; ++w) {
         const uint32_t k = wide[w];
         k_pauli_apply<<<g1, kB, 0, st>>>(s->adj_phi, s->adj_lam, size, masks[k].x, masks[k].z, masks[k].ny,
@@ -330,5 +370,154 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
     uint64_t j = 0;
     for (uint64_t i = 0; i < num_ops; ++i)
         if (ops[i].param >= 0) grad[ops[i].param] += host[1 + j++];
+    return KBWH_OK;
+}
+
+// Parameter sets in one sweep: the nsets states lie one after another, set
+// k's 2^n amplitudes at k 2^n, so together they are one state of
+// n + ceil(log2 nsets) qubits whose top bits select the set.  Every gate acts
+// below qubit n, so each tiled pass (8 or more tiles per set) runs over all
+// sets at once with each set's own gate records; lambda = H phi is the same
+// Hamiltonian on every set; energies and gradient terms are reduced per set.
+extern "C" int32_t kbwh_adjoint_batch(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                                      uint32_t num_params, uint32_t num_sets, const uint8_t *paulis,
+                                      const uint32_t *qubits, const uint32_t *offsets, const double *coeffs,
+                                      uint32_t num_strings, double *energies, double *grads) {
+    if (!s || !energies || num_sets == 0 || (num_ops && !ops) || (num_params && (!params || !grads)) ||
+        (num_strings && (!offsets || !coeffs)))
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    FLUSH(s);
+    if (s->block) return set_error(KBWH_ERR_INVALID_ARGUMENT, "adjoint gradient needs a flat state");
+    const uint32_t n = s->n;
+    if (n < 11) {
+        // states smaller than a tile: one gradient per set
+        for (uint32_t k = 0; k < num_sets; ++k) {
+            const int32_t rc = kbwh_adjoint_gradient(s, ops, num_ops, params + (uint64_t)k * num_params, num_params,
+                                                     paulis, qubits, offsets, coeffs, num_strings, energies + k,
+                                                     grads ? grads + (uint64_t)k * num_params : nullptr);
+            if (rc) return rc;
+        }
+        return KBWH_OK;
+    }
+    if (num_sets > 65535) return set_error(KBWH_ERR_INVALID_ARGUMENT, "at most 65535 parameter sets");
+    for (uint64_t i = 0; i < num_ops; ++i) {
+        const kbwh_op &op = ops[i];
+        if (op.kind > KRZ || op.target >= n || (n < 64 && (op.ctrl_mask >> n)))
+            return set_error(KBWH_ERR_INVALID_ARGUMENT, "bad op");
+        if (op.param >= 0) {
+            if ((uint32_t)op.param >= num_params)
+                return set_error(KBWH_ERR_INVALID_ARGUMENT, "param index out of range");
+            if (!(op.kind == KRX || op.kind == KRY || op.kind == KRZ || op.kind == KP))
+                return set_error(KBWH_ERR_INVALID_ARGUMENT, "parameterised ops must be RX, RY, RZ or P");
+        }
+    }
+    std::vector<PauliMasks> masks(num_strings);
+    for (uint32_t k = 0; k < num_strings; ++k) {
+        const int32_t rc = pauli_masks(s, paulis, qubits, offsets[k], offsets[k + 1], false, &masks[k]);
+        if (rc) return rc;
+    }
+    uint32_t b = 0;
+    while ((1u << b) < num_sets) ++b;
+    if (n + b + 1 > 40) return set_error(KBWH_ERR_INVALID_ARGUMENT, "parameter sets too large for one sweep");
+    const uint64_t set_size = s->size, span = 1ull << (n + b);
+    const uint64_t total = (uint64_t)num_sets * set_size;
+    uint64_t nparam_ops = 0;
+    for (uint64_t i = 0; i < num_ops; ++i) nparam_ops += ops[i].param >= 0;
+    kbwh_set_buffers &sb = s->sets;
+    int32_t rc = grow_device(s, &sb.vecs, &sb.vecs_cap, 2 * span * sizeof(double2));
+    if (rc) return rc;
+    // device doubles: the parameters, the energies, the gradient terms, the energy partials
+    constexpr uint32_t kEB = 16;
+    const uint64_t np_all = (uint64_t)num_sets * num_params;
+    const uint64_t nslots = (uint64_t)num_sets * nparam_ops;
+    if ((rc = grow_device(s, &sb.params, &sb.params_cap,
+                          (np_all + num_sets + nslots + (uint64_t)num_sets * kEB + 1) * sizeof(double))))
+        return rc;
+    double *d_params = (double *)sb.params, *d_energy = d_params + np_all, *d_slots = d_energy + num_sets;
+    double *d_epart = d_slots + nslots;
+    double2 *phi = (double2 *)sb.vecs, *lam = phi + span;
+    hipStream_t st = s->stream;
+    hipError_t e = np_all ? hipMemcpyAsync(d_params, params, np_all * sizeof(double), hipMemcpyHostToDevice, st)
+                          : hipSuccess;
+    if (e == hipSuccess) {
+        k_init_sets<<<grid_for(total, kB), kB, 0, st>>>(phi, total, set_size - 1);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return hip_status(e, "adjoint sets init");
+    // forward: phi_k = U(params_k) |0>
+    if ((rc = run_sets_forward(s, phi, n, num_sets, ops, num_ops, d_params, num_params, sb))) return rc;
+    s->adj_forward = s->last_passes;
+    s->adj_lambda = 0;
+    s->adj_reverse = 0;
+    // lambda = H phi on every set, then E_k = Re <phi_k|lambda_k>
+    std::vector<PauliMasks> narrow;
+    std::vector<double> narrow_coeffs;
+    std::vector<uint32_t> wide;
+    for (uint32_t k = 0; k < num_strings; ++k) {
+        if (__builtin_popcountll(masks[k].x | 7ull) <= 11) {
+            narrow.push_back(masks[k]);
+            narrow_coeffs.push_back(coeffs[k]);
+        } else {
+            wide.push_back(k);
+        }
+    }
+    s->adj_lambda = wide.size();
+    bool assigned = false;
+    if (!narrow.empty()) {
+        uint32_t sweeps = 0;
+        rc = pauli_apply_tiled(s, phi, lam, narrow, narrow_coeffs.data(), n, (uint64_t)num_sets << (n - 11),
+                               &sweeps);
+        if (rc) return rc;
+        s->adj_lambda += sweeps;
+        assigned = true;
+    }
+    const uint32_t gt = grid_for(total, kB);
+    if (num_strings == 0) {
+        k_pauli_apply<<<gt, kB, 0, st>>>(phi, lam, total, 0, 0, 0, 0.0, 1);  // H = 0: lambda = 0
+        e = hipGetLastError();
+    }
+    for (size_t w = 0; w < wide.size() && e == hipSuccess; ++w) {
+        const uint32_t k = wide[w];
+        k_pauli_apply<<<gt, kB, 0, st>>>(phi, lam, total, masks[k].x, masks[k].z, masks[k].ny, coeffs[k], !assigned);
+        e = hipGetLastError();
+        assigned = true;
+    }
+    if (e == hipSuccess) {
+        k_dot_sets<<<dim3(kEB, num_sets), kB, 0, st>>>(phi, lam, set_size, d_epart);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        k_sum_sets<<<num_sets, kB, 0, st>>>(d_epart, kEB, d_energy);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return hip_status(e, "adjoint sets lambda");
+    // reverse sweep over all sets
+    if (nparam_ops && (rc = adjoint_reverse_sets(s, phi, n, num_sets, n + b, ops, num_ops, d_params, num_params,
+                                                 d_slots)))
+        return rc;
+    std::vector<double> host(num_sets + nslots);
+    e = hipMemcpyAsync(host.data(), d_energy, host.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_status(e, "adjoint sets read-back");
+    for (uint32_t k = 0; k < num_sets; ++k) {
+        energies[k] = host[k];
+        if (!grads) continue;
+        double *g = grads + (uint64_t)k * num_params;
+        for (uint32_t j = 0; j < num_params; ++j) g[j] = 0.0;
+        const double *t = host.data() + num_sets + (uint64_t)k * nparam_ops;
+        uint64_t j = 0;
+        for (uint64_t i = 0; i < num_ops; ++i)
+            if (ops[i].param >= 0) g[ops[i].param] += t[j++];
+    }
+    return KBWH_OK;
+}
+
+extern "C" int32_t kbwh_adjoint_stats(const kbwh_state *s, uint64_t *forward_passes, uint64_t *lambda_sweeps,
+                                      uint64_t *reverse_passes) {
+    if (!s || !forward_passes || !lambda_sweeps || !reverse_passes)
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    *forward_passes = s->adj_forward;
+    *lambda_sweeps = s->adj_lambda;
+    *reverse_passes = s->adj_reverse;
     return KBWH_OK;
 }

@@ -113,20 +113,26 @@ __device__ __forceinline__ void phi_partners(const dv2 (&v)[TR], dv2 (&o)[kHalf]
     }
 }
 
-// v: the tile of the (n + 1)-qubit state, phi in slots r < kHalf and lambda in r + kHalf
+// v: the tile of the state with lambda's selector qubit, phi in slots
+// r < kHalf and lambda in r + kHalf.  Parameter sets: workgroup w of set
+// (blockIdx = set * wps + w) walks tiles set * tps + w, + wps, ... with the
+// set's own gate records gs + set * ng (one set: wps = gridDim.x).
 __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec ts, const AGate *__restrict__ gs,
-                                                 uint32_t ng, uint32_t ntiles, uint32_t nslots,
+                                                 uint32_t ng, uint32_t tps, uint32_t wps, uint32_t nslots,
                                                  double *__restrict__ partial) {
     __shared__ dv2 lds[TT * kChunk];
     __shared__ double acc[kMaxSlots * TT];
     __shared__ double ws[TT / 64];
     const uint32_t tid = threadIdx.x;
+    const uint32_t set = blockIdx.x / wps, wk = blockIdx.x % wps;
+    gs += (size_t)set * ng;
     for (uint32_t s = 0; s < nslots; ++s) acc[s * TT + tid] = 0.0;
     uint64_t tmask = 0;
 #pragma unroll
     for (int b = 0; b < TB; ++b)
         if ((tid >> b) & 1u) tmask |= ts.qbit[b];
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (uint32_t tl = wk; tl < tps; tl += wps) {
+        const uint32_t t = set * tps + tl;
         const uint64_t base = tile_base(t, ts);
         const uint64_t dt = base | tmask;
         dv2 v[TR];
@@ -183,32 +189,50 @@ __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec t
     }
 }
 
-// out[map.global[s]] = the sum of slot s's workgroup partials, in order
-__global__ __launch_bounds__(256) void k_sum_slots(const double *__restrict__ partial, uint32_t grid,
-                                                  double *__restrict__ out, SlotMap map) {
+// out[set * ostride + map.global[s]] = the sum, in order, of slot s's
+// partials from the wps workgroups of set (blockIdx.x = s, blockIdx.y = set)
+__global__ __launch_bounds__(256) void k_sum_slots(const double *__restrict__ partial, uint32_t grid, uint32_t wps,
+                                                  uint64_t ostride, double *__restrict__ out, SlotMap map) {
     __shared__ double ws[4];
-    const uint32_t s = blockIdx.x;
+    const uint32_t s = blockIdx.x, set = blockIdx.y;
+    const double *p = partial + (size_t)s * grid + (size_t)set * wps;
     double a = 0.0;
-    for (uint32_t i = threadIdx.x; i < grid; i += 256) a += partial[(size_t)s * grid + i];
+    for (uint32_t i = threadIdx.x; i < wps; i += 256) a += p[i];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) a += __shfl_xor(a, off, 64);
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = a;
     __syncthreads();
-    if (threadIdx.x == 0) out[map.global[s]] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+    if (threadIdx.x == 0) out[set * ostride + map.global[s]] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
-// Tile positions of a pass: qubits 0..2 on lane bits 0..2, qubit n (lambda)
-// on register bit 11, the most used targets on register bits 8..10, then lane
+// AGate records of S parameter sets from one set's templates (as
+// k_expand_sets of the forward passes): the inverse gate's coefficients from
+// -params[set][pidx[r]] when pidx[r] >= 0
+__global__ __launch_bounds__(256) void k_expand_asets(const AGate *__restrict__ tmpl, const int32_t *__restrict__ pidx,
+                                                      const uint32_t *__restrict__ dst,
+                                                      const uint32_t *__restrict__ stride, uint32_t nrec,
+                                                      uint32_t nsets, const double *__restrict__ params,
+                                                      uint32_t nparams, AGate *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint64_t)nrec * nsets) return;
+    const uint32_t set = (uint32_t)(i / nrec), r = (uint32_t)(i % nrec);
+    AGate g = tmpl[r];
+    if (pidx[r] >= 0) set_coefficients_dev(g.inv, -params[(uint64_t)set * nparams + (uint32_t)pidx[r]]);
+    out[dst[r] + (uint64_t)set * stride[r]] = g;
+}
+
+// Tile positions of a pass: qubits 0..2 on lane bits 0..2, lambda's selector
+// qubit (n for one state) on register bit 11, the most used targets on register bits 8..10, then lane
 // bits 3..5, then wave bits 6..7; free positions take the lowest unused qubits.
-void assign_adj(uint32_t n, const Pass &p, int pos_of[64], uint32_t q_at[TL]) {
+void assign_adj(uint32_t n, uint32_t lam_qubit, const Pass &p, int pos_of[64], uint32_t q_at[TL]) {
     for (int i = 0; i < 64; ++i) pos_of[i] = -1;
     for (int b = 0; b < TL; ++b) q_at[b] = 0xFFFFFFFFu;
     for (uint32_t q = 0; q < 3; ++q) {
         q_at[q] = q;
         pos_of[q] = (int)q;
     }
-    q_at[TL - 1] = n;
-    pos_of[n] = TL - 1;
+    q_at[TL - 1] = lam_qubit;
+    pos_of[lam_qubit] = TL - 1;
     std::vector<std::pair<int, uint32_t>> freq;
     for (uint32_t t : p.targets) {
         if (t < 3) continue;
@@ -238,33 +262,36 @@ void assign_adj(uint32_t n, const Pass &p, int pos_of[64], uint32_t q_at[TL]) {
 
 }  // namespace
 
-// The reverse sweep over phi and lambda (s->adj_phi, lambda right after it)
-// in tiled passes; slots[j] receives the term of the j-th parameterised op
-// (forward order).  Needs n >= 11.
-int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
-                              double *slots, double *partial, size_t partial_cap) {
-    const uint32_t n = s->n;
-    if (n < 11 || n > 62) return set_error(KBWH_ERR_INVALID_ARGUMENT, "tiled reverse sweep needs 11..62 qubits");
+namespace {
+
+// The reverse sweep's passes for one set: gate tables and slot maps (the
+// coefficients of parameterised gates are those of `params`, or left for the
+// device when params is null).
+struct RPass {
+    TileSpec ts;
+    uint32_t first, count, nslots;
+    SlotMap map;
+};
+
+int32_t plan_reverse(uint32_t n, uint32_t lam_qubit, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                     std::vector<RPass> &passes, std::vector<AGate> &table, std::vector<int32_t> &pidx) {
     uint64_t nparam = 0;
     for (uint64_t i = 0; i < num_ops; ++i) nparam += ops[i].param >= 0;
-    struct RPass {
-        TileSpec ts;
-        uint32_t first, count, nslots;
-        SlotMap map;
-    };
     // the reversed list (each gate at -angle for the rotations); rev[j] is
     // ops[num_ops - 1 - j], param_of[j] its index among the parameterised ops
     std::vector<PhysOp> rev(num_ops);
     std::vector<uint8_t> is_param(num_ops);
     std::vector<uint32_t> param_of(num_ops, kNoSlot);
+    std::vector<int32_t> param_idx(num_ops, -1);
     {
         uint64_t global = nparam;
         for (uint64_t j = 0; j < num_ops; ++j) {
             const kbwh_op &op = ops[num_ops - 1 - j];
-            const double angle = (params && op.param >= 0) ? params[op.param] : op.angle;
+            const double angle = (params && op.param >= 0) ? params[op.param] : (op.param >= 0 ? 0.0 : op.angle);
             const bool rot = op.kind == KRX || op.kind == KRY || op.kind == KRZ || op.kind == KP;
             rev[j] = PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), rot ? -angle : angle};
             is_param[j] = op.param >= 0;
+            param_idx[j] = op.param;
             if (is_param[j]) param_of[j] = (uint32_t)--global;
         }
     }
@@ -274,15 +301,14 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
     // list may be scheduled like KBWH_RUN_REORDER, kMaxSlots terms per pass
     std::vector<uint32_t> order, ends;
     schedule_passes(rev, n, kTargets, order, ends, &is_param, kMaxSlots);
-    std::vector<RPass> passes;
-    std::vector<AGate> table;
+    if (order.size() != num_ops) return set_error(KBWH_ERR_DEVICE, "adjoint schedule incomplete");
     size_t k = 0;
     for (const uint32_t end : ends) {
         Pass cur;
         for (size_t i = k; i < end; ++i) add_op(cur, rev[order[i]]);
         int pos_of[64];
         uint32_t q_at[TL];
-        assign_adj(n, cur, pos_of, q_at);
+        assign_adj(n, lam_qubit, cur, pos_of, q_at);
         RPass P;
         memset(&P.map, 0, sizeof P.map);
         P.ts = make_tile_spec(q_at);
@@ -293,7 +319,7 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
             const uint32_t j = order[i];
             AGate A;
             memset(&A, 0, sizeof A);
-            A.inv = make_tgate(rev[j], pos_of, n + 1);
+            A.inv = make_tgate(rev[j], pos_of, n);
             A.gkind = (uint32_t)rev[j].kind;
             A.slot = kNoSlot;
             if (is_param[j]) {
@@ -301,11 +327,29 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
                 P.map.global[P.nslots++] = param_of[j];
             }
             table.push_back(A);
+            pidx.push_back(param_idx[j]);
         }
         if (P.nslots > kMaxSlots) return set_error(KBWH_ERR_DEVICE, "adjoint pass over its slots");
         passes.push_back(P);
         k = end;
     }
+    return KBWH_OK;
+}
+
+}  // namespace
+
+// The reverse sweep over phi and lambda (s->adj_phi, lambda right after it)// The reverse sweep over phi and lambda (s->adj_phi, lambda right after it)
+// in tiled passes; slots[j] receives the term of the j-th parameterised op
+// (forward order).  Needs n >= 11.
+int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                              double *slots, double *partial, size_t partial_cap) {
+    const uint32_t n = s->n;
+    if (n < 11 || n > 62) return set_error(KBWH_ERR_INVALID_ARGUMENT, "tiled reverse sweep needs 11..62 qubits");
+    std::vector<RPass> passes;
+    std::vector<AGate> table;
+    std::vector<int32_t> pidx;
+    int32_t prc = plan_reverse(n, n, ops, num_ops, params, passes, table, pidx);
+    if (prc) return prc;
     const uint32_t ntiles = (uint32_t)(1ull << (n - 11));
     const uint32_t grid = std::min<uint32_t>(ntiles, kAdjTiledGrid);
     if ((size_t)grid * kMaxSlots > partial_cap) return set_error(KBWH_ERR_DEVICE, "adjoint scratch too small");
@@ -324,17 +368,83 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
     const AGate *dg = (const AGate *)s->tile_gates;
     for (const RPass &P : passes) {
         if (e != hipSuccess) break;
-        k_adj_tile<<<grid, TT, 0, s->stream>>>((dv2 *)s->adj_phi, P.ts, dg + P.first, P.count, ntiles, P.nslots,
-                                               partial);
+        k_adj_tile<<<grid, TT, 0, s->stream>>>((dv2 *)s->adj_phi, P.ts, dg + P.first, P.count, ntiles, grid,
+                                               P.nslots, partial);
         e = hipGetLastError();
         if (e == hipSuccess && P.nslots) {
-            k_sum_slots<<<P.nslots, 256, 0, s->stream>>>(partial, grid, slots, P.map);
+            k_sum_slots<<<dim3(P.nslots, 1), 256, 0, s->stream>>>(partial, grid, grid, 0, slots, P.map);
             e = hipGetLastError();
         }
     }
+    s->adj_reverse = passes.size();
     // the host gate table is local: the upload must be done before it goes away
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     return hip_status(e, "adjoint reverse passes");
+}
+
+int32_t adjoint_reverse_sets(kbwh_state *s, double2 *phi, uint32_t n, uint32_t nsets, uint32_t lam_qubit,
+                             const kbwh_op *ops, uint64_t num_ops, const double *params_dev, uint32_t num_params,
+                             double *slots) {
+    if (n < 11 || n > 40 || lam_qubit < n || lam_qubit > 62)
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "tiled reverse sweep over sets needs 11..40 qubits");
+    std::vector<RPass> passes;
+    std::vector<AGate> table;
+    std::vector<int32_t> pidx;
+    int32_t rc = plan_reverse(n, lam_qubit, ops, num_ops, nullptr, passes, table, pidx);
+    if (rc) return rc;
+    uint64_t nparam = 0;
+    for (uint64_t i = 0; i < num_ops; ++i) nparam += ops[i].param >= 0;
+    const size_t nrec = table.size();
+    if (nrec == 0) return KBWH_OK;
+    // wps workgroups per set share the set's tiles (about kAdjTiledGrid in all)
+    const uint32_t tps = 1u << (n - 11);
+    const uint32_t wps = std::max<uint32_t>(1, std::min<uint32_t>(tps, kAdjTiledGrid / nsets));
+    const uint64_t grid = (uint64_t)nsets * wps;
+    if (grid > 0x7FFFFFFFull || (uint64_t)nrec * nsets > 0xFFFFFFFFull)
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "too many parameter sets");
+    kbwh_set_buffers &sb = s->sets;
+    if ((rc = grow_device(s, &sb.partial, &sb.partial_cap, grid * kMaxSlots * sizeof(double)))) return rc;
+    if ((rc = grow_device(s, &sb.tmpl, &sb.tmpl_cap, nrec * (sizeof(AGate) + 12)))) return rc;
+    if ((rc = grow_device(s, &sb.gates, &sb.gates_cap, nrec * nsets * sizeof(AGate)))) return rc;
+    std::vector<uint32_t> dst(nrec), stride(nrec);
+    for (const RPass &P : passes)
+        for (uint32_t i = 0; i < P.count; ++i) {
+            dst[P.first + i] = (uint32_t)((uint64_t)nsets * P.first + i);
+            stride[P.first + i] = P.count;
+        }
+    char *tb = (char *)sb.tmpl;
+    AGate *d_tmpl = (AGate *)tb;
+    int32_t *d_pidx = (int32_t *)(tb + nrec * sizeof(AGate));
+    uint32_t *d_dst = (uint32_t *)(d_pidx + nrec), *d_stride = d_dst + nrec;
+    hipStream_t st = s->stream;
+    hipError_t e = hipMemcpyAsync(d_tmpl, table.data(), nrec * sizeof(AGate), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pidx, pidx.data(), nrec * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_dst, dst.data(), nrec * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_stride, stride.data(), nrec * 4, hipMemcpyHostToDevice, st);
+    AGate *dg = (AGate *)sb.gates;
+    if (e == hipSuccess) {
+        const uint64_t total = (uint64_t)nrec * nsets;
+        k_expand_asets<<<(uint32_t)((total + 255) / 256), 256, 0, st>>>(d_tmpl, d_pidx, d_dst, d_stride,
+                                                                        (uint32_t)nrec, nsets, params_dev,
+                                                                        num_params, dg);
+        e = hipGetLastError();
+    }
+    double *partial = (double *)sb.partial;
+    for (const RPass &P : passes) {
+        if (e != hipSuccess) break;
+        k_adj_tile<<<(uint32_t)grid, TT, 0, st>>>((dv2 *)phi, P.ts, dg + (size_t)nsets * P.first, P.count, tps, wps,
+                                                  P.nslots, partial);
+        e = hipGetLastError();
+        if (e == hipSuccess && P.nslots) {
+            k_sum_slots<<<dim3(P.nslots, nsets), 256, 0, st>>>(partial, (uint32_t)grid, wps, nparam, slots, P.map);
+            e = hipGetLastError();
+        }
+    }
+    s->adj_reverse = passes.size();
+    // the host tables are local: the uploads must be done before they go away
+    const hipError_t f = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = f;
+    return hip_status(e, "adjoint reverse passes over sets");
 }
 
 }  // namespace kbwh

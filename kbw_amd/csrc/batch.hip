@@ -86,37 +86,7 @@ __global__ __launch_bounds__(256) void k_coef(const TGate *__restrict__ tmpl, TG
     if (i >= ngates) return;
     TGate g = tmpl[i];
     const int32_t p = (int32_t)g.pad;
-    if (p >= 0) {
-        const double th = params[p];
-        double s, c;
-        switch (g.kind) {
-        case KRX:
-            sincos(th / 2.0, &s, &c);
-            g.au = c;
-            g.cru = s;
-            g.ciu = -s;
-            break;
-        case KRY:
-            sincos(th / 2.0, &s, &c);
-            g.au = c;
-            g.bs = s;
-            break;
-        case KP:
-            sincos(th, &s, &c);
-            g.ph0x = 1.0;
-            g.ph1x = 1.0 * c;
-            g.ph1y = 1.0 * s;
-            break;
-        default:  // KRZ
-            sincos(-th / 2.0, &s, &c);
-            g.ph0x = 1.0 * c;
-            g.ph0y = 1.0 * s;
-            sincos(th / 2.0, &s, &c);
-            g.ph1x = 1.0 * c;
-            g.ph1y = 1.0 * s;
-            break;
-        }
-    }
+    if (p >= 0) set_coefficients_dev(g, params[p]);
     live[i] = g;
 }
 

@@ -23,6 +23,21 @@ struct kbwh_phys_event {
     uint32_t swap_local, swap_global;
 };
 
+// Device buffers of a batched adjoint gradient (kbwh_adjoint_batch), grown
+// as needed and freed with the handle.
+struct kbwh_set_buffers {
+    void *vecs = nullptr;  // phi and lambda of every set
+    size_t vecs_cap = 0;
+    void *gates = nullptr;  // expanded gate records of every set
+    size_t gates_cap = 0;
+    void *tmpl = nullptr;  // templates, parameter indices, destinations, strides
+    size_t tmpl_cap = 0;
+    void *params = nullptr;  // the sets' parameters, then energies and gradient terms
+    size_t params_cap = 0;
+    void *partial = nullptr;  // reduction partials
+    size_t partial_cap = 0;
+};
+
 struct kbwh_state {
     uint32_t n = 0;
     bool block = false;
@@ -62,6 +77,10 @@ struct kbwh_state {
     double2 *adj_lam = nullptr;
     double *adj_partial = nullptr;
     size_t adj_partial_cap = 0;
+    kbwh_set_buffers sets;  // batched adjoint gradients
+    // the last adjoint gradient's HBM passes: forward passes over phi, sweeps
+    // building lambda = H phi, reverse passes over phi and lambda
+    uint64_t adj_forward = 0, adj_lambda = 0, adj_reverse = 0;
 };
 
 namespace kbwh {
@@ -86,8 +105,10 @@ int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const dou
 constexpr size_t kTileSpecBytes = 176, kTGateBytes = 144;
 hipError_t launch_tile_exact(double2 *amp, const void *spec, const void *store, const void *gates,
                              uint32_t num_gates, uint32_t ntiles, hipStream_t stream);
+// set_shift < 32: tile t uses the records gates + (t >> set_shift) * num_gates
+// (parameter sets laid out one after another, see run_sets_forward).
 hipError_t launch_tile_contracted(double2 *amp, const void *spec, const void *gates, uint32_t num_gates,
-                                  uint32_t ntiles, hipStream_t stream);
+                                  uint32_t ntiles, hipStream_t stream, uint32_t set_shift = 32);
 // reorder: dependency-ordered passes (KBWH_RUN_REORDER; flat states only)
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
                      const double *params, bool reorder = false);
@@ -99,6 +120,13 @@ constexpr uint32_t kAdjTiledSlots = 24;  // parameterised gates per pass
 constexpr size_t kAdjTiledScratch = (size_t)kAdjTiledGrid * kAdjTiledSlots;
 int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
                               double *slots, double *partial, size_t partial_cap);
+// The reverse sweep over nsets parameter sets (n >= 11): phi holds set k's
+// 2^n amplitudes at k 2^n, lambda lies 2^lam_qubit amplitudes after phi;
+// slots[k * (parameterised ops) + j] receives set k's term of the j-th
+// parameterised op.  params_dev: nsets x num_params on the device.
+int32_t adjoint_reverse_sets(kbwh_state *s, double2 *phi, uint32_t n, uint32_t nsets, uint32_t lam_qubit,
+                             const kbwh_op *ops, uint64_t num_ops, const double *params_dev, uint32_t num_params,
+                             double *slots);
 int32_t queue_gate(kbwh_state *s, int kind, double angle, uint32_t target, uint64_t ctrl_mask);
 int32_t flush_queue(kbwh_state *s);
 bool queue_pending(const kbwh_state *s);
@@ -131,8 +159,20 @@ int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const 
 // Copies count doubles of device results (stream-ordered) to the host through
 // the handle's pinned buffer.
 int32_t reduce_from(kbwh_state *s, const double *src, double *out, uint32_t count);
+// lam = sum_k coeffs[k] P_k phi in grouped tile sweeps.  n: the qubits the
+// strings act on (0: the handle's); ntiles_total: tiles to sweep (0: all of
+// an n-qubit state; parameter sets one after another have more).
 int32_t pauli_apply_tiled(kbwh_state *s, const double2 *phi, double2 *lam, const std::vector<PauliMasks> &ms,
-                          const double *coeffs);
+                          const double *coeffs, uint32_t n = 0, uint64_t ntiles_total = 0, uint32_t *sweeps = nullptr);
+// Forward run of a circuit over nsets parameter sets laid out one after
+// another in amp (set k's 2^n amplitudes at k 2^n): dependency-ordered tiled
+// passes over the circuit's qubits (n >= 11), each set with its own angles
+// params_dev[k * num_params + op.param] (a device array).
+int32_t run_sets_forward(kbwh_state *s, double2 *amp, uint32_t n, uint32_t nsets, const kbwh_op *ops,
+                         uint64_t num_ops, const double *params_dev, uint32_t num_params, kbwh_set_buffers &sb);
+// Grows a device buffer to at least bytes (stream-ordered: waits for the
+// handle's stream before freeing the old one).
+int32_t grow_device(kbwh_state *s, void **buf, size_t *cap, size_t bytes);
 int32_t pauli_masks(const kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
                     uint32_t lo, uint32_t hi, bool physical, PauliMasks *out);
 

@@ -60,6 +60,19 @@ int32_t hip_status(hipError_t e, const char *what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
+int32_t grow_device(kbwh_state *s, void **buf, size_t *cap, size_t bytes) {
+    if (*cap >= bytes) return KBWH_OK;
+    if (*buf) {
+        HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipFree(*buf));
+    }
+    *buf = nullptr;
+    *cap = 0;
+    HIPCHK(hipMalloc(buf, bytes));
+    *cap = bytes;
+    return KBWH_OK;
+}
+
 int32_t ensure_scratch(kbwh_state *s, size_t doubles) {
     if (s->scratch_cap >= doubles) return KBWH_OK;
     if (s->scratch) HIPCHK(hipFree(s->scratch));

@@ -673,6 +673,41 @@ inline void set_coefficients(TGate &g, int kind, double angle) {
     }
 }
 
+// The coefficients of a parameterised RX, RY, RZ or P record from its angle,
+// on the device: the forms of set_coefficients with the device library's
+// sincos, so they agree with the host's libm values to rounding.
+__device__ __forceinline__ void set_coefficients_dev(TGate &g, double th) {
+    double s, c;
+    switch (g.kind) {
+    case KRX:
+        sincos(th / 2.0, &s, &c);
+        g.au = c;
+        g.cru = s;
+        g.ciu = -s;
+        break;
+    case KRY:
+        sincos(th / 2.0, &s, &c);
+        g.au = c;
+        g.bs = s;
+        break;
+    case KP:
+        sincos(th, &s, &c);
+        g.ph0x = 1.0;
+        g.ph0y = 0.0;
+        g.ph1x = c;
+        g.ph1y = s;
+        break;
+    default:  // KRZ
+        sincos(-th / 2.0, &s, &c);
+        g.ph0x = c;
+        g.ph0y = s;
+        sincos(th / 2.0, &s, &c);
+        g.ph1x = c;
+        g.ph1y = s;
+        break;
+    }
+}
+
 // The device record of one gate of a pass, relative to the pass's tile positions.
 inline TGate make_tgate(const PhysOp &o, const int pos_of[64], uint32_t n) {
     TGate g;

@@ -31,14 +31,18 @@ using namespace tile;
 // record through scalar registers on every gate.
 constexpr uint32_t kPrefetchBelowTiles = 1024;
 
+// Sets: a state that holds several independent states of the same circuit
+// (parameter sets) one after another takes each set's own gate records:
+// tile t reads gates + (t >> set_shift) * ng (set_shift >= 32: one set).
 template <bool PERM, bool PF>
 __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void KBWH_TILE_PASS_NAME(
     dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles,
-    TileSpec st) {
+    TileSpec st, uint32_t set_shift) {
     __shared__ dv2 lds[TT * kChunk];
     const uint32_t tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
     if (tile >= ntiles) return;
+    if (set_shift < 32) gates += (size_t)(tile >> set_shift) * ng;
     uint64_t tmask = 0;
 #pragma unroll
     for (int b = 0; b < TB; ++b)

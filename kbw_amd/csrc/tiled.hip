@@ -164,6 +164,23 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     for (int r = 0; r < TR; ++r) lam[slot_offset(dt, r, ts)] = out[r];
 }
 
+// Gate records of S parameter sets from one set's templates: record r of
+// the templates becomes out[dst[r] + set * stride[r]] (a pass's records for
+// set 0, then set 1, ...), with the coefficients of pidx[r]'s angle in that
+// set, times sign (-1: the inverse gate), when pidx[r] >= 0.
+__global__ __launch_bounds__(256) void k_expand_sets(const TGate *__restrict__ tmpl, const int32_t *__restrict__ pidx,
+                                                     const uint32_t *__restrict__ dst,
+                                                     const uint32_t *__restrict__ stride, uint32_t nrec,
+                                                     uint32_t nsets, const double *__restrict__ params,
+                                                     uint32_t nparams, double sign, TGate *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint64_t)nrec * nsets) return;
+    const uint32_t set = (uint32_t)(i / nrec), r = (uint32_t)(i % nrec);
+    TGate g = tmpl[r];
+    if (pidx[r] >= 0) set_coefficients_dev(g, sign * params[(uint64_t)set * nparams + (uint32_t)pidx[r]]);
+    out[dst[r] + (uint64_t)set * stride[r]] = g;
+}
+
 struct SGroup {
     TileSpec ts;
     uint32_t first, count, has_x;
@@ -292,17 +309,18 @@ int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const 
 // lam = sum_k coeffs[k] P_k phi for vectors of s->n >= TL qubits, in grouped
 // sweeps (one read of phi and one read-modify-write of lam per group).
 int32_t pauli_apply_tiled(kbwh_state *s, const double2 *phi, double2 *lam, const std::vector<PauliMasks> &ms,
-                          const double *coeffs) {
-    const uint32_t n = s->n;
+                          const double *coeffs, uint32_t n, uint64_t ntiles_total, uint32_t *sweeps) {
+    if (n == 0) n = s->n;
     if (n < (uint32_t)TL) return set_error(KBWH_ERR_INVALID_ARGUMENT, "grouped Pauli sums need at least 11 qubits");
     std::vector<SGroup> groups;
     std::vector<SString> strings;
     int32_t rc = group_strings(n, ms, coeffs, true, groups, strings);
     if (rc) return rc;
     if (groups.empty()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "no Pauli strings");
+    if (sweeps) *sweeps = (uint32_t)groups.size();
     rc = upload_strings(s, strings);
     if (rc) return rc;
-    const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
+    const uint32_t ntiles = ntiles_total ? (uint32_t)ntiles_total : (uint32_t)(1ull << (n - TL));
     const SString *ds = (const SString *)s->tile_gates;
     hipError_t e = hipSuccess;
     for (size_t gi = 0; gi < groups.size() && e == hipSuccess; ++gi) {
@@ -530,6 +548,87 @@ int32_t check_ops(uint32_t n, const kbwh_op *ops, uint64_t num_ops) {
 
 int32_t run_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params) {
     return run_tiled_on(s, s->amp, ops, num_ops, params, false);
+}
+
+int32_t run_sets_forward(kbwh_state *s, double2 *amp, uint32_t n, uint32_t nsets, const kbwh_op *ops,
+                         uint64_t num_ops, const double *params_dev, uint32_t num_params, kbwh_set_buffers &sb) {
+    if (n < (uint32_t)TL || n > 40) return set_error(KBWH_ERR_INVALID_ARGUMENT, "parameter sets need 11..40 qubits");
+    if (nsets == 0 || ((uint64_t)nsets << (n - TL)) > 0x7FFFFFFFull)
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "parameter set count out of range");
+    int32_t rc = check_ops(n, ops, num_ops);
+    if (rc) return rc;
+    // the structure is the same in every set: parameterised gates get their
+    // coefficients on the device (k_expand_sets); no diagonal fusion, whose
+    // choices depend on the angles
+    std::vector<PhysOp> phys(num_ops);
+    for (uint64_t i = 0; i < num_ops; ++i) {
+        const kbwh_op &op = ops[i];
+        phys[i] = PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), op.param >= 0 ? 0.0 : op.angle};
+    }
+    std::vector<uint32_t> order, ends;
+    schedule_passes(phys, n, std::min<uint32_t>(s->tile_targets, TL - 3), order, ends);
+    if (order.size() != num_ops) return set_error(KBWH_ERR_DEVICE, "set schedule incomplete");
+    struct SL {
+        TileSpec ts;
+        uint32_t first, count;
+    };
+    std::vector<SL> launches;
+    std::vector<TGate> tmpl;
+    std::vector<int32_t> pidx;
+    std::vector<uint32_t> dst, stride;
+    size_t k = 0;
+    for (const uint32_t end : ends) {
+        Pass cur;
+        for (size_t i = k; i < end; ++i) add_op(cur, phys[order[i]]);
+        int pos_of[64];
+        uint32_t q_at[TL];
+        assign_positions(n, cur, pos_of, q_at);
+        SL L;
+        L.ts = make_tile_spec(q_at);
+        L.first = (uint32_t)tmpl.size();
+        L.count = (uint32_t)(end - k);
+        for (size_t i = k; i < end; ++i) {
+            tmpl.push_back(make_tgate(phys[order[i]], pos_of, n));
+            pidx.push_back(ops[order[i]].param);
+            dst.push_back((uint32_t)((uint64_t)nsets * L.first + (i - k)));
+            stride.push_back(L.count);
+        }
+        launches.push_back(L);
+        k = end;
+    }
+    const size_t nrec = tmpl.size();
+    if (nrec == 0) return KBWH_OK;
+    if ((uint64_t)nrec * nsets > 0xFFFFFFFFull) return set_error(KBWH_ERR_INVALID_ARGUMENT, "too many gate records");
+    const size_t tbytes = nrec * (sizeof(TGate) + 12);
+    if ((rc = grow_device(s, &sb.tmpl, &sb.tmpl_cap, tbytes))) return rc;
+    if ((rc = grow_device(s, &sb.gates, &sb.gates_cap, nrec * nsets * sizeof(TGate)))) return rc;
+    char *tb = (char *)sb.tmpl;
+    TGate *d_tmpl = (TGate *)tb;
+    int32_t *d_pidx = (int32_t *)(tb + nrec * sizeof(TGate));
+    uint32_t *d_dst = (uint32_t *)(d_pidx + nrec), *d_stride = d_dst + nrec;
+    hipError_t e = hipMemcpyAsync(d_tmpl, tmpl.data(), nrec * sizeof(TGate), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_pidx, pidx.data(), nrec * 4, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_dst, dst.data(), nrec * 4, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_stride, stride.data(), nrec * 4, hipMemcpyHostToDevice, s->stream);
+    TGate *d_gates = (TGate *)sb.gates;
+    if (e == hipSuccess) {
+        const uint64_t total = (uint64_t)nrec * nsets;
+        k_expand_sets<<<(uint32_t)((total + 255) / 256), 256, 0, s->stream>>>(
+            d_tmpl, d_pidx, d_dst, d_stride, (uint32_t)nrec, nsets, params_dev, num_params, 1.0, d_gates);
+        e = hipGetLastError();
+    }
+    const uint32_t ntiles = nsets << (n - TL);
+    for (size_t i = 0; i < launches.size() && e == hipSuccess; ++i) {
+        const SL &L = launches[i];
+        e = launch_tile_contracted(amp, &L.ts, d_gates + (size_t)nsets * L.first, L.count, ntiles, s->stream,
+                                   n - TL);
+    }
+    s->last_passes = launches.size();
+    s->last_gates = num_ops;
+    // the host tables are local: the uploads must be done before they go away
+    const hipError_t f = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = f;
+    return hip_status(e, "parameter-set passes");
 }
 
 int32_t run_events(kbwh_state *s, double2 *amp, const std::vector<kbwh_phys_event> &ev) {

@@ -16,16 +16,18 @@
 namespace kbwh {
 
 hipError_t launch_tile_contracted(double2 *amp, const void *spec, const void *gates, uint32_t num_gates,
-                                  uint32_t ntiles, hipStream_t stream) {
+                                  uint32_t ntiles, hipStream_t stream, uint32_t set_shift) {
     static_assert(sizeof(tile::TileSpec) == kTileSpecBytes && sizeof(tile::TGate) == kTGateBytes,
                   "tile records shared with tiled.hip");
     tile::TileSpec ts;
     memcpy(&ts, spec, sizeof ts);
     const tile::TGate *g = (const tile::TGate *)gates;
     if (ntiles < kPrefetchBelowTiles)
-        k_tile_fma<false, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts);
+        k_tile_fma<false, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts,
+                                                                 set_shift);
     else
-        k_tile_fma<false, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts);
+        k_tile_fma<false, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts,
+                                                                  set_shift);
     return hipGetLastError();
 }
 

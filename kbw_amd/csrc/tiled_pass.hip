@@ -23,13 +23,13 @@ hipError_t launch_tile_exact(double2 *amp, const void *spec, const void *store, 
         tile::TileSpec st;
         memcpy(&st, store, sizeof st);
         if (pf)
-            k_tile<true, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, st);
+            k_tile<true, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, st, 32u);
         else
-            k_tile<true, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, st);
+            k_tile<true, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, st, 32u);
     } else if (pf) {
-        k_tile<false, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts);
+        k_tile<false, true><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts, 32u);
     } else {
-        k_tile<false, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts);
+        k_tile<false, false><<<ntiles, tile::TT, 0, stream>>>((tile::dv2 *)amp, ts, g, num_gates, ntiles, ts, 32u);
     }
     return hipGetLastError();
 }

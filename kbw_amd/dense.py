@@ -240,6 +240,26 @@ class DenseHip:
                                               C.byref(e), ptr(grad)))
         return e.value, grad
 
+    def adjoint_batch(self, ops, param_sets, hamiltonian):
+        """Energies and gradients for many parameter sets of one circuit
+        (rows of param_sets) in one sweep over all sets."""
+        a = ops_array(ops)
+        p = np.ascontiguousarray(np.atleast_2d(param_sets), dtype=np.float64)
+        nsets, nparams = p.shape
+        paulis, qubits, offsets, coeffs = hamiltonian.flatten()
+        energies = np.zeros(nsets, dtype=np.float64)
+        grads = np.zeros((nsets, nparams), dtype=np.float64)
+        check(self._lib.kbwh_adjoint_batch(self._h, ptr(a), len(a), ptr(p), nparams, nsets, ptr(paulis),
+                                           ptr(qubits), ptr(offsets), ptr(coeffs), len(coeffs), ptr(energies),
+                                           ptr(grads)))
+        return energies, grads
+
+    def adjoint_stats(self):
+        """(forward passes, lambda sweeps, reverse passes) of the last adjoint gradient."""
+        f, l, r = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(self._lib.kbwh_adjoint_stats(self._h, C.byref(f), C.byref(l), C.byref(r)))
+        return f.value, l.value, r.value
+
     # ---- raw access ---------------------------------------------------
     def state(self, first=0, count=None):
         """Physical-order amplitudes as complex128 (a host copy)."""

@@ -42,15 +42,21 @@ def test_tiled_brickwork_bit_exact(n, layers):
     assert_same(g.state(), o.state())
 
 
+@pytest.mark.parametrize("mode", ["tiled", "reorder"])
 @pytest.mark.parametrize("kind", range(8))
-def test_tiled_every_kind_at_every_tile_position(kind):
+def test_tiled_every_kind_at_every_tile_position(kind, mode):
     """One gate of `kind` at each tile position: lane bits 0..5, wave bits 6..7,
     register bits 8..10; uncontrolled, with a control on a lane bit, with one
     on a register bit and with one outside the tile, on a random state.  In a
     pass whose first-used targets are qubits 3..10 (each used once), the
     planner gives them positions 8, 9, 10, 3, 4, 5, 6, 7 in that order; qubits
-    0..2 are always positions 0..2.  The other targets get X gates."""
-    print(f"tiled kind {kind} at every position", flush=True)
+    0..2 are always positions 0..2.  The other targets get X gates.
+    Dependency-ordered runs (the contracted kernel's in-place forms) give the
+    targets some bijection onto the same positions, so every position is
+    covered too; they agree with the per-gate path to 1e-13."""
+    from kbw_amd import RUN_REORDER
+
+    print(f"tiled kind {kind} at every position, {mode}", flush=True)
     n = 13
     rng = np.random.default_rng(40 + kind)
     st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
@@ -59,26 +65,33 @@ def test_tiled_every_kind_at_every_tile_position(kind):
     for test_q in range(11):
         lane_c = [0] if test_q != 0 else [1]
         reg_c = [3] if test_q != 3 else [4]
-        for ctrl in ([], lane_c, reg_c, [12]):
+        # angles whose half-angle (or phase) has cos > 0 and |cos| > |sin|,
+        # cos > 0 and |sin| > |cos|, cos < 0 and |sin| > |cos|, cos < 0 and
+        # |cos| > |sin|: every shear and lane form of the in-place kernel
+        for ctrl, ang in zip(([], lane_c, reg_c, [12]), (0.9, 2.5, 4.0, 5.9)):
             ops = circuits.OpList()
             if test_q < 3:
-                ops.add(kind, test_q, ctrl, angle=0.9)
+                ops.add(kind, test_q, ctrl, angle=ang)
             for t in range(3, 11):
                 if t == test_q:
-                    ops.add(kind, t, ctrl, angle=0.9)
+                    ops.add(kind, t, ctrl, angle=ang)
                 else:
                     ops.add(0, t)
             cases.append((test_q, ctrl, ops.array()))
     for test_q, ctrl, ops in cases:
         a = DenseHip(n)
         a.set_state(st)
-        a.run(ops, mode=RUN_TILED)
+        a.run(ops, mode=RUN_TILED if mode == "tiled" else RUN_REORDER)
         assert a.run_stats()[1] == 1
         b = DenseHip(n)
         b.set_state(st)
         b.run(ops, mode=RUN_PER_GATE)
-        bad = np.nonzero(a.state() != b.state())[0]
-        assert bad.size == 0, (kind, test_q, ctrl, bad.size)
+        if mode == "tiled":
+            bad = np.nonzero(a.state() != b.state())[0]
+            assert bad.size == 0, (kind, test_q, ctrl, bad.size)
+        else:
+            d = float(np.max(np.abs(a.state() - b.state())))
+            assert d <= 1e-13, (kind, test_q, ctrl, d)
         a.close()
         b.close()
 

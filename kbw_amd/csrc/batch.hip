@@ -15,11 +15,15 @@
 //               parameter vector (device sincos), once per run;
 //   k_reset  -- the work counter back to 0 (a kernel, not a memset, so the
 //               run can be captured in a hipGraph);
-//   k_batch  -- persistent: every workgroup takes whole circuits from the
-//               counter, largest first, and runs the circuit's passes over
-//               all of its tiles (the first pass makes |0...0> in registers
-//               instead of loading); a workgroup exits when the counter runs
-//               past the last circuit, so every wave reaches the exit.
+//   k_batch  -- persistent: every workgroup takes work items from the
+//               counter, largest circuits first.  An item is a chunk of a
+//               circuit's tiles (a circuit of up to 32 tiles is one chunk,
+//               larger ones up to 16 chunks of about 32 tiles); the item runs
+//               every pass of its circuit over its tiles (the first pass
+//               makes |0...0> in registers instead of loading), and the
+//               chunks of one circuit meet at a barrier between passes.  A
+//               workgroup exits when the counter runs past the last item, so
+//               every wave reaches the exit.
 // With graphs enabled, the parameter upload, the three kernels and the result
 // download are captured once and replayed by later runs.
 #include <math.h>
@@ -31,6 +35,9 @@
 
 #include "internal.h"
 #include "state.h"
+// batch energies agree with the reference to rounding (angles come from the
+// device's sincos), so gate products may fuse into FMAs
+#define KBWH_TILE_CONTRACT 1
 #include "tile_core.h"
 
 #pragma clang fp contract(off)
@@ -52,6 +59,25 @@ struct BCirc {
     uint32_t num_strings;
     uint32_t pad[2];
 };
+
+// One work item: tiles [first_tile, first_tile + num_tiles) of circuit circ
+// in every pass; the circuit's chunks are consecutive items.
+struct BItem {
+    uint32_t circ, chunk, nchunks;
+    uint32_t first_tile, num_tiles;
+    uint32_t pad[3];
+};
+
+// Chunks: a circuit of more than kChunkTiles tiles is split into up to
+// kMaxChunks chunks.  The persistent grid must hold at least kMaxChunks
+// resident workgroups (it holds 4 per CU), so the chunks of a circuit that
+// has started are running or are the next items a free workgroup takes.
+constexpr uint32_t kChunkTiles = 32;
+constexpr uint32_t kMaxChunks = 16;
+// A barrier wait gives up after this many polls (about a second): the
+// workgroup then goes on, the error flag is set and the host reports it, so
+// the kernel ends even if a chunk never arrived.
+constexpr uint32_t kSpinLimit = 1u << 24;
 
 struct BPass {
     TileSpec ts;
@@ -90,12 +116,42 @@ __global__ __launch_bounds__(256) void k_coef(const TGate *__restrict__ tmpl, TG
     live[i] = g;
 }
 
-__global__ void k_reset(uint32_t *counter) { *counter = 0; }
+// Zeroes the control words before a run (a kernel, not a memset, so the run
+// can be captured in a hipGraph): [0] item counter, [1] error flag, then a
+// barrier counter and a done counter per circuit.
+__global__ __launch_bounds__(256) void k_reset(uint32_t *ctl, uint32_t count) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) ctl[i] = 0;
+}
+
+// All chunks of a circuit have finished pass p before any starts pass p + 1:
+// the workgroup's stores are made visible device-wide, thread 0 arrives and
+// polls until every chunk of the circuit has arrived for this pass (target =
+// (p + 1) * chunks), and the wait ends with an acquire, so the next pass
+// reads what the other chunks wrote.
+__device__ __forceinline__ void circuit_barrier(uint32_t *bar, uint32_t target, uint32_t *err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(bar, 1u);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(4);
+            if (++spins >= kSpinLimit) {
+                atomicOr(err, 1u);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
 
 __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_batch(
-    dv2 *__restrict__ states, const BCirc *__restrict__ circs, const uint32_t *__restrict__ order,
-    uint32_t ncirc, const BPass *__restrict__ passes, const TGate *__restrict__ gates,
-    const BString *__restrict__ strings, double *__restrict__ results, uint32_t *__restrict__ counter) {
+    dv2 *__restrict__ states, const BCirc *__restrict__ circs, const BItem *__restrict__ items,
+    uint32_t nitems, const BPass *__restrict__ passes, const TGate *__restrict__ gates,
+    const BString *__restrict__ strings, double *__restrict__ results, double *__restrict__ partial,
+    uint32_t *__restrict__ ctl, uint32_t ncirc) {
+    uint32_t *const counter = ctl, *const err = ctl + 1, *const bar = ctl + 2, *const done = ctl + 2 + ncirc;
     __shared__ dv2 lds[TT * TR];  // gate exchanges use the first TT * kChunk slots
     __shared__ double ws[TT / 64];
     __shared__ uint32_t s_item;
@@ -110,11 +166,12 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     for (;;) {
         const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
-        if (item >= ncirc) break;
-        const uint32_t ci = order[item];
+        if (item >= nitems) break;
+        const BItem I = items[item];
+        const uint32_t ci = I.circ;
         const BCirc C = circs[ci];
         dv2 *a = states + C.state_off;
-        const uint32_t ntiles = 1u << (C.n_eff - TL);
+        const uint32_t t_end = I.first_tile + I.num_tiles;
         double total = 0.0;
         for (uint32_t p = 0; p < C.num_passes; ++p) {
             const BPass &P = passes[C.first_pass + p];
@@ -123,7 +180,7 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
 #pragma unroll
             for (int b = 0; b < TB; ++b)
                 if ((tid >> b) & 1u) tmask |= P.ts.qbit[b];
-            for (uint32_t t = 0; t < ntiles; ++t) {
+            for (uint32_t t = I.first_tile; t < t_end; ++t) {
                 const uint64_t base = tile_base(t, P.ts);
                 const uint64_t dt = base | tmask;
                 dv2 v[TR];
@@ -198,9 +255,25 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
             }
             __threadfence_block();
             __syncthreads();
+            if (I.nchunks > 1 && p + 1 < C.num_passes) circuit_barrier(bar + ci, (p + 1) * I.nchunks, err);
         }
         if (tid == 0) {
-            results[ci] = total;
+            if (I.nchunks == 1) {
+                results[ci] = total;
+            } else {
+                // the last chunk to finish sums the chunks' energies in chunk
+                // order, so the result does not depend on which one that is
+                partial[item] = total;
+                __threadfence();
+                if (atomicAdd(done + ci, 1u) + 1 == I.nchunks) {
+                    __threadfence();
+                    const uint32_t first = item - I.chunk;
+                    double sum = 0.0;
+                    for (uint32_t j = 0; j < I.nchunks; ++j)
+                        sum += __hip_atomic_load(partial + first + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    results[ci] = sum;
+                }
+            }
             s_item = atomicAdd(counter, 1u);
         }
         __syncthreads();
@@ -216,13 +289,15 @@ struct kbwh_batch {
     hipStream_t stream = nullptr;
     dv2 *states = nullptr;
     BCirc *d_circs = nullptr;
-    uint32_t *d_order = nullptr;
+    BItem *d_items = nullptr;
+    uint32_t nitems = 0;
+    double *d_partial = nullptr;  // chunk energies, one per item
     BPass *d_passes = nullptr;
     TGate *d_tmpl = nullptr, *d_live = nullptr;
     BString *d_strings = nullptr;
     double *d_params = nullptr, *d_results = nullptr;
-    uint32_t *d_counter = nullptr;
-    double *h_params = nullptr, *h_results = nullptr;  // pinned
+    uint32_t *d_ctl = nullptr;  // item counter, error flag, per-circuit barrier and done counters
+    double *h_params = nullptr, *h_results = nullptr;  // pinned; h_results ends with the error flag
     std::vector<BCirc> circs;
     uint32_t npasses = 0;
     bool use_graph = false;
@@ -242,17 +317,20 @@ int32_t enqueue_run(kbwh_batch *b) {
         k_coef<<<(uint32_t)((b->ngates + 255) / 256), 256, 0, st>>>(b->d_tmpl, b->d_live, b->ngates, b->d_params);
         e = hipGetLastError();
     }
+    const uint32_t nctl = 2 + 2 * b->ncirc;
     if (e == hipSuccess) {
-        k_reset<<<1, 1, 0, st>>>(b->d_counter);
+        k_reset<<<std::min<uint32_t>((nctl + 255) / 256, 64), 256, 0, st>>>(b->d_ctl, nctl);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
-        k_batch<<<b->grid, TT, 0, st>>>(b->states, b->d_circs, b->d_order, b->ncirc, b->d_passes, b->d_live,
-                                       b->d_strings, b->d_results, b->d_counter);
+        k_batch<<<b->grid, TT, 0, st>>>(b->states, b->d_circs, b->d_items, b->nitems, b->d_passes, b->d_live,
+                                       b->d_strings, b->d_results, b->d_partial, b->d_ctl, b->ncirc);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->h_results, b->d_results, b->ncirc * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->h_results + b->ncirc, b->d_ctl + 1, 4, hipMemcpyDeviceToHost, st);
     return hip_status(e, "batch run");
 }
 
@@ -261,8 +339,8 @@ void destroy(kbwh_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void *dev[] = {b->states, b->d_circs, b->d_order, b->d_passes, b->d_tmpl, b->d_live,
-                   b->d_strings, b->d_params, b->d_results, b->d_counter};
+    void *dev[] = {b->states, b->d_circs, b->d_items, b->d_partial, b->d_passes, b->d_tmpl, b->d_live,
+                   b->d_strings, b->d_params, b->d_results, b->d_ctl};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (b->h_params) (void)hipHostFree(b->h_params);
@@ -517,20 +595,41 @@ int32_t kbwh_batch_new(const kbwh_circuit *circuits, uint32_t num_circuits, cons
     int dev = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    b->grid = (uint32_t)std::max(1, std::min<int>((int)num_circuits, std::max(cus, 1) * 4));
+    // persistent grid: 4 resident workgroups per CU (launch bounds of 4 waves
+    // per SIMD, 32 KiB of LDS each)
+    const uint32_t resident = (uint32_t)std::max(cus, 1) * 4;
+    const uint32_t max_chunks = std::min<uint32_t>(kMaxChunks, resident);
+    std::vector<BItem> items;
+    for (const uint32_t c : order) {
+        const uint32_t ntiles = 1u << (circs[c].n_eff - TL);
+        const uint32_t nch = std::max<uint32_t>(1, std::min<uint32_t>(max_chunks, ntiles / kChunkTiles));
+        for (uint32_t j = 0; j < nch; ++j) {
+            BItem I;
+            memset(&I, 0, sizeof I);
+            I.circ = c;
+            I.chunk = j;
+            I.nchunks = nch;
+            I.first_tile = (uint32_t)((uint64_t)ntiles * j / nch);
+            I.num_tiles = (uint32_t)((uint64_t)ntiles * (j + 1) / nch) - I.first_tile;
+            items.push_back(I);
+        }
+    }
+    b->nitems = (uint32_t)items.size();
+    b->grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(items.size(), resident));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&b->states, std::max<uint64_t>(state_total, 1) * sizeof(dv2));
     if (e == hipSuccess) e = upload(&b->d_circs, circs);
-    if (e == hipSuccess) e = upload(&b->d_order, order);
+    if (e == hipSuccess) e = upload(&b->d_items, items);
+    if (e == hipSuccess) e = hipMalloc(&b->d_partial, std::max<size_t>(items.size(), 1) * 8);
     if (e == hipSuccess) e = upload(&b->d_passes, passes);
     if (e == hipSuccess) e = upload(&b->d_tmpl, tmpl);
     if (e == hipSuccess) e = hipMalloc(&b->d_live, std::max<size_t>(tmpl.size(), 1) * sizeof(TGate));
     if (e == hipSuccess) e = upload(&b->d_strings, strings);
     if (e == hipSuccess) e = hipMalloc(&b->d_params, std::max<uint64_t>(num_params, 1) * 8);
     if (e == hipSuccess) e = hipMalloc(&b->d_results, std::max<uint32_t>(num_circuits, 1) * 8);
-    if (e == hipSuccess) e = hipMalloc(&b->d_counter, 64);
+    if (e == hipSuccess) e = hipMalloc(&b->d_ctl, (2 + 2 * (size_t)num_circuits) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(&b->h_params, std::max<uint64_t>(num_params, 1) * 8, 0);
-    if (e == hipSuccess) e = hipHostMalloc(&b->h_results, std::max<uint32_t>(num_circuits, 1) * 8, 0);
+    if (e == hipSuccess) e = hipHostMalloc(&b->h_results, ((size_t)num_circuits + 1) * 8, 0);
     if (e != hipSuccess) {
         int32_t rc = hip_status(e, "kbwh_batch_new");
         destroy(b);
@@ -565,6 +664,9 @@ int32_t kbwh_batch_run(kbwh_batch *b, const double *params, double *exp_values) 
     }
     e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_status(e, "batch sync");
+    uint32_t flag = 0;
+    memcpy(&flag, b->h_results + b->ncirc, 4);
+    if (flag) return set_error(KBWH_ERR_DEVICE, "batch: a circuit's chunks did not meet at a pass barrier in time");
     memcpy(exp_values, b->h_results, b->ncirc * 8);
     return KBWH_OK;
 }

@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--mode", choices=["per_gate", "tiled", "best"], default="best")
     ap.add_argument("--cpu-gates", type=int, default=150, help="oracle sample size (gates); 0 = skip")
     ap.add_argument("--legs", default="tiled,reorder,per_gate,exp,adjoint,batch,live,cpu",
-                    help="comma list of legs: tiled, reorder, per_gate, exp, adjoint, batch, live, cpu, c2, fused")
+                    help="comma list of legs: tiled, reorder, per_gate (with its per-kind, per-target table), "
+                         "pergate_table, exp, adjoint, batch, live, cpu, c2, fused")
     ap.add_argument("--pmc", action="store_true",
                     help="counter run: a 1 GiB torch copy then 4 pair passes (for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -486,6 +487,44 @@ def cpu_baseline(n, ops, ngates):
             "host": host_info()}
 
 
+def pergate_table_leg(sim, stream, n, reps=3):
+    """The plain per-gate path (one kbwh_gate_mask launch over the state) for
+    X, H, RY, RZ, P and CZ on every target: median launch time over `reps`
+    launches (HIP events on the engine's stream) and the fraction of 8 TB/s
+    achieved on the gate's algorithmic bytes (read + write of each touched
+    amplitude: 32 B x 2^n, 2^(n-1) for P, 2^(n-2) for CZ).  Z and P below
+    bit 3 (and CZ there) touch part of every 128-byte line, so whole lines
+    move and their fraction is capped near 0.5 (0.25 for CZ) of the
+    streaming rate."""
+    import numpy as np
+    import torch
+
+    from kbw_amd import _lib, circuits
+
+    lib, h = sim._lib, sim.handle
+    kinds = {"X": (_lib.X, 0.0), "H": (_lib.H, 0.0), "RY": (_lib.RY, 0.7), "RZ": (_lib.RZ, 0.7),
+             "P": (_lib.P, 0.7), "CZ": (_lib.Z, 0.0)}
+    table = {}
+    for name, (kind, ang) in kinds.items():
+        row = []
+        for t in range(n):
+            mask = (1 << (t + 1 if t + 1 < n else t - 1)) if name == "CZ" else 0
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+            ev[0].record(stream)
+            for r in range(reps):
+                _lib.check(lib.kbwh_gate_mask(h, kind, ang, t, mask))
+                ev[r + 1].record(stream)
+            sim.synchronize()
+            ms = float(np.median([ev[r].elapsed_time(ev[r + 1]) for r in range(reps)]))
+            gb = circuits.gate_bytes(n, kind, mask, t) / (ms * 1e-3) / 1e9
+            row.append(round(gb / HBM_PEAK_GBPS, 4))
+        table[name] = row
+    summary = {k: {"mean_frac": float(np.mean(v)), "min_frac": float(np.min(v)),
+                   "min_target": int(np.argmin(v))} for k, v in table.items()}
+    return {"frac_by_kind_and_target": table, "summary": summary, "reps": reps,
+            "note": "fraction of 8 TB/s on algorithmic bytes per launch; CZ's control is the target's neighbour"}
+
+
 def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12):
     """The same circuit in dependency-ordered tiled passes (RUN_REORDER):
     from |0>, the state is compared with the program-order tiled run over all
@@ -620,6 +659,9 @@ def main():
                            "kernel": "k_pair<KH> (uncontrolled H pass)", "launch_ms": pair_ms,
                            "bytes_per_launch": per_launch, "launches": int(is_pair.sum()) * pg_steps}}
         result["per_gate"] = pg
+    if "pergate_table" in legs or (want_pg and "per_gate" in legs):
+        pg_table = pergate_table_leg(sim, stream, n)
+        result.setdefault("per_gate", {})["by_kind_and_target"] = pg_table
 
     if want_tiled:
         torch.cuda.synchronize()

@@ -116,6 +116,48 @@ __global__ __launch_bounds__(kBlock) void k_diag(double2 *__restrict__ a, uint64
         }
 }
 
+// Low targets (pair gates below bit 6, Z and P below bit 3; also RZ over
+// more than 2^31 amplitudes, which this kernel's grid stride covers): each lane
+// takes one amplitude, so every wave reads and writes 64 consecutive
+// amplitudes (1 KiB), and gets the other amplitude of its pair from the lane
+// across the target bit (a cross-lane shuffle).  The pair is computed with
+// the reference's arithmetic, and each lane keeps its own half, so the
+// results are the same bits as k_pair / k_diag.  Controls at bit 6 and above
+// are inserted into the index (only control-satisfied 64-amplitude segments
+// are visited); controls below bit 6 are tested per lane (lo_ctrl).  Z and P
+// on bits 0..2 touch every other amplitude of each 128-byte line, so the
+// whole line is read and written back either way; streaming whole lines
+// avoids partial-line writes.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_low(double2 *__restrict__ a, uint64_t count, uint32_t t,
+                                                uint64_t lo_ctrl, uint64_t set_hi, Insert ins_hi, double c,
+                                                double s, double2 ph0, double2 ph1) {
+    // count and the stride are multiples of 64, so a wave is either wholly
+    // in range or wholly out (the shuffle needs all 64 lanes)
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < count; k += stride) {
+        const uint64_t i = expand(k, ins_hi) | set_hi;
+        double2 v = ldnt(&a[i]);
+        const bool one = ((i >> t) & 1u) != 0;
+        const bool ok = (i & lo_ctrl) == lo_ctrl;
+        if constexpr (KIND == KZ || KIND == KP) {
+            if (ok && one) v = KIND == KZ ? cmul(v, make_double2(-1.0, -0.0)) : cmul(v, ph1);  // dense_v2.rs:191, 216
+        } else if constexpr (KIND == KRZ) {
+            if (ok) v = cmul(v, one ? ph1 : ph0);  // dense_v2.rs:271-272
+        } else {
+            // component-wise selects keep the pair in registers (a select of
+            // whole double2 objects is lowered to a select of addresses)
+            const int lm = 1 << t;
+            const double ox = __shfl_xor(v.x, lm, 64), oy = __shfl_xor(v.y, lm, 64);
+            double2 a0 = make_double2(one ? ox : v.x, one ? oy : v.y);
+            double2 a1 = make_double2(one ? v.x : ox, one ? v.y : oy);
+            pair_math<KIND>(a0, a1, c, s);
+            if (ok) v = make_double2(one ? a1.x : a0.x, one ? a1.y : a0.y);
+        }
+        stnt(&a[i], v);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_init(double2 *__restrict__ a, uint64_t size) {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < size; i += stride)
@@ -426,11 +468,35 @@ hipError_t launch_gate(double2 *a, uint32_t n, int kind, double angle, uint32_t 
     ctrl_mask &= ~tbit;  // a control equal to the target is dropped (dense_v2.rs:56-57)
     const uint32_t c = (uint32_t)__builtin_popcountll(ctrl_mask);
     const Coef k = make_coef(kind, angle);
-    if (kind == KRZ) {
+    // RZ over more than 2^31 amplitudes goes to the grid-striding kernel below
+    if (kind == KRZ && n - c <= 31) {
         const uint64_t count = 1ull << (n - c);
         const Insert ins = make_insert(ctrl_mask);
         k_diag<KRZ><<<grid_for(count, kBlock * kPPT), kBlock, 0, st>>>(a, count, tbit, ctrl_mask,
                                                                         ins, k.ph0, k.ph1);
+        return hipGetLastError();
+    }
+    // low targets: one amplitude per lane, the pair's partner from another
+    // lane (needs whole 64-amplitude segments: the controls at bit 6 and up
+    // leave at least 6 free bits)
+    const uint64_t hi_ctrl = ctrl_mask & ~63ull, lo_ctrl = ctrl_mask & 63ull;
+    const uint32_t c_hi = (uint32_t)__builtin_popcountll(hi_ctrl);
+    const bool diag = kind == KZ || kind == KP;
+    if ((kind == KRZ || target < (diag ? 3u : 6u)) && n >= 6 + c_hi) {
+        const uint64_t cnt = 1ull << (n - c_hi);
+        const Insert ins = make_insert(hi_ctrl);
+        const uint32_t g = (uint32_t)std::min<uint64_t>(grid_for(cnt, kBlock), 1u << 22);  // 2^30 lanes at most
+        switch (kind) {
+        case KX: k_low<KX><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KY: k_low<KY><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KZ: k_low<KZ><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KH: k_low<KH><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KP: k_low<KP><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KRX: k_low<KRX><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KRY: k_low<KRY><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KRZ: k_low<KRZ><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        default: return hipErrorInvalidValue;
+        }
         return hipGetLastError();
     }
     const uint64_t count = 1ull << (n - 1 - c);

@@ -62,6 +62,59 @@ def test_single_gates_every_target_bit_exact(n, kind):
     assert_same(g, o)
 
 
+@pytest.mark.parametrize("n", [6, 7, 8, 12])
+def test_low_target_gates_with_low_and_high_controls(n):
+    """Targets 0..5 (one amplitude per lane, partner by cross-lane shuffle)
+    under controls below bit 6 (tested per lane), at bit 6 and above
+    (inserted into the index) and both; n=7 and 8 with every high bit a
+    control leave exactly one 64-amplitude segment per launch."""
+    print(f"low-target gates n={n}", flush=True)
+    rng = np.random.default_rng(500 + n)
+    g, o = pair(n, random_state(rng, n))
+    high = list(range(6, n))
+    for kind in range(8):
+        for t in range(6):
+            low = [q for q in range(6) if q != t]
+            sets = [[], [low[0]], low[1:3], high[:1], high, [low[2]] + high[-2:]]
+            for ctrl in sets:
+                ang = float(rng.uniform(-7, 7))
+                g.gate(kind, ang, t, ctrl)
+                o.gate(kind, ang, t, ctrl)
+    assert_same(g, o)
+
+
+def test_32_qubit_gates_over_more_than_2_31_amplitudes():
+    """At 32 qubits an uncontrolled RZ spans 2^32 amplitudes and a low-target
+    X 2^32 lanes: both run as grid-striding loops.  Two 4096-amplitude
+    windows of the state are set (bits 0..11 vary; the second has bits 14, 17
+    and 31 set) and checked bit-exactly against a 15-qubit oracle holding the
+    same windows, with oracle qubits 12, 13, 14 standing for bits 14, 17, 31."""
+    print("32-qubit gates", flush=True)
+    n = 32
+    rng = np.random.default_rng(32)
+    wa, wb = random_state(rng, 12), random_state(rng, 12)
+    first_b = (1 << 31) | (1 << 17) | (1 << 14)
+    g = DenseHip(n)
+    g.set_state(wa, first=0)
+    g.set_state(wb, first=first_b)
+    o = oracle.DenseV2(15)
+    psi = np.zeros(1 << 15, dtype=complex)
+    base_b = 0b111 << 12
+    psi[:4096], psi[base_b:base_b + 4096] = wa, wb
+    o.set_state(psi)
+    gates = [(RZ, 0.7, 0, []), (RZ, -1.9, 31, []), (RZ, 2.3, 17, [3]),
+             (X, 0.0, 2, []), (RY, 1.1, 5, [1, 14]), (H, 0.0, 0, [])]
+    where = {31: 14, 17: 13, 14: 12}
+    for kind, ang, t, ctrl in gates:
+        g.gate(kind, ang, t, ctrl)
+        o.gate(kind, ang, where.get(t, t), [where.get(c, c) for c in ctrl])
+    ref = o.state()
+    assert np.array_equal(g.state(0, 4096), ref[:4096])
+    assert np.array_equal(g.state(first_b, 4096), ref[base_b:base_b + 4096])
+    assert not np.any(g.state(1 << 20, 4096))
+    g.close()
+
+
 @pytest.mark.parametrize("n,ngates,seed", [(4, 300, 0), (10, 400, 1), (15, 300, 2), (20, 120, 3)])
 def test_random_circuits_bit_exact(n, ngates, seed):
     print(f"random circuit n={n}", flush=True)

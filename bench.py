@@ -43,6 +43,7 @@ HBM_PEAK_GBPS = 8000.0
 TRAFFIC_PAIR = 34362315175.0
 TRAFFIC_TILE = 34369246667.0
 TRAFFIC_TILE_FMA = 34359974941.0
+TRAFFIC_COMPILED = None  # not yet counted
 
 
 def parse():
@@ -54,8 +55,8 @@ def parse():
     ap.add_argument("--layers", type=int, default=20)
     ap.add_argument("--mode", choices=["per_gate", "tiled", "best"], default="best")
     ap.add_argument("--cpu-gates", type=int, default=150, help="oracle sample size (gates); 0 = skip")
-    ap.add_argument("--legs", default="tiled,reorder,per_gate,exp,adjoint,batch,live,cpu",
-                    help="comma list of legs: tiled, reorder, per_gate (with its per-kind, per-target table), "
+    ap.add_argument("--legs", default="tiled,reorder,compiled,per_gate,exp,adjoint,batch,live,cpu",
+                    help="comma list of legs: tiled, reorder, compiled, per_gate (with its per-kind, per-target table), "
                          "pergate_table, exp, adjoint, batch, live, cpu, c2, fused")
     ap.add_argument("--pmc", action="store_true",
                     help="counter run: a 1 GiB torch copy then 4 pair passes (for rocprofv3 --pmc)")
@@ -65,10 +66,10 @@ def parse():
 def pmc_run(n):
     """A 1 GiB device copy (the counter scale), 4 uncontrolled pair passes,
     the first tiled passes of the brickwork circuit (k_tile) and the same
-    gates in dependency-ordered passes (k_tile_fma)."""
+    gates in dependency-ordered passes (k_tile_fma, then compiled kernels)."""
     import torch
 
-    from kbw_amd import DenseHip, RUN_REORDER, RUN_TILED, circuits
+    from kbw_amd import DenseHip, RUN_COMPILED, RUN_REORDER, RUN_TILED, circuits
 
     a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
     a.fill_(1.0)
@@ -84,8 +85,12 @@ def pmc_run(n):
     tiled_passes = s.run_stats()[1]
     s.run(ops[:120], mode=RUN_REORDER)
     s.synchronize()
+    reorder_passes = s.run_stats()[1]
+    s.run(ops[:120], mode=RUN_COMPILED)
+    s.synchronize()
     print(json.dumps({"pmc_run": True, "copy_bytes": 1 << 30, "pass_bytes": 32 << n,
-                      "tiled_passes": tiled_passes, "reorder_passes": s.run_stats()[1]}), flush=True)
+                      "tiled_passes": tiled_passes, "reorder_passes": reorder_passes,
+                      "compiled_passes": s.run_stats()[1]}), flush=True)
 
 
 def block_leg(shots=1_000_000):
@@ -525,27 +530,36 @@ def pergate_table_leg(sim, stream, n, reps=3):
             "note": "fraction of 8 TB/s on algorithmic bytes per launch; CZ's control is the target's neighbour"}
 
 
-def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12):
-    """The same circuit in dependency-ordered tiled passes (RUN_REORDER):
+def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12, compiled=False):
+    """The same circuit in dependency-ordered tiled passes (RUN_REORDER, or
+    RUN_COMPILED: the same passes, each run by a kernel compiled for it):
     from |0>, the state is compared with the program-order tiled run over all
-    2^n amplitudes (max |difference| <= tol), then K steps are timed."""
+    2^n amplitudes (max |difference| <= tol), then K steps are timed.  The
+    compiled mode's kernels are compiled by its first run (the comparison
+    run); their compile time is reported, not timed."""
     import numpy as np
     import torch
 
-    from kbw_amd import DenseHip, _lib
+    from kbw_amd import DenseHip, _lib, compiled_stats
 
+    mode = _lib.RUN_COMPILED if compiled else _lib.RUN_REORDER
     nops = len(ops)
     r = DenseHip(n)
     rstream = torch.cuda.ExternalStream(r.stream())
     sim.clear()
     sim.run(ops, mode=_lib.RUN_TILED)
-    r.run(ops, mode=_lib.RUN_REORDER)
+    before = compiled_stats()
+    t_first = time.perf_counter()
+    r.run(ops, mode=mode)
+    r.synchronize()
+    t_first = time.perf_counter() - t_first
+    after = compiled_stats()
     chunk = 1 << min(n, 26)
     worst = 0.0
     for first in range(0, 1 << n, chunk):
         worst = max(worst, float(np.max(np.abs(r.state(first, chunk) - sim.state(first, chunk)))))
     for _ in range(args.warmup):
-        r.run(ops, mode=_lib.RUN_REORDER)
+        r.run(ops, mode=mode)
     r.synchronize()
     torch.cuda.synchronize()
     evs = []
@@ -553,7 +567,7 @@ def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12):
     for _ in range(args.steps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(rstream)
-        r.run(ops, mode=_lib.RUN_REORDER)
+        r.run(ops, mode=mode)
         e1.record(rstream)
         evs.append((e0, e1))
     r.synchronize()
@@ -564,14 +578,25 @@ def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12):
     launch_ms = run_ms / passes
     ach = per_launch / (launch_ms * 1e-3) / 1e9
     r.close()
-    return {"gates_per_s": nops * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
-            "passes_per_step": passes, "max_abs_diff_vs_program_order": worst, "tolerance": tol,
-            "within_tolerance": worst <= tol,
-            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBPS, "traffic": TRAFFIC_TILE_FMA,
-                         "kernel": "k_tile_fma (11-qubit register tiles, dependency-ordered passes)",
-                         "launch_ms": launch_ms, "bytes_per_launch": per_launch,
-                         "launches": passes * args.steps}}
+    out = {"gates_per_s": nops * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
+           "passes_per_step": passes, "max_abs_diff_vs_program_order": worst, "tolerance": tol,
+           "within_tolerance": worst <= tol,
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBPS, "traffic": TRAFFIC_TILE_FMA,
+                        "kernel": "k_tile_fma (11-qubit register tiles, dependency-ordered passes)",
+                        "launch_ms": launch_ms, "bytes_per_launch": per_launch,
+                        "launches": passes * args.steps}}
+    if compiled:
+        out["roofline"].update({"traffic": TRAFFIC_COMPILED,
+                                "kernel": "kbwh_pass_* (a kernel compiled per dependency-ordered pass, "
+                                          "11-qubit register tiles)"})
+        out["compile"] = {"kernels": after["compiled"] - before["compiled"],
+                          "failures": after["failures"] - before["failures"],
+                          "generic_passes": after["generic_passes"] - before["generic_passes"],
+                          "compile_s": after["compile_seconds"] - before["compile_seconds"],
+                          "first_run_s": t_first}
+        out["within_tolerance"] = out["within_tolerance"] and out["compile"]["failures"] == 0
+    return out
 
 
 def main():
@@ -687,17 +712,19 @@ def main():
         result["tiled"] = {"gates_per_s": result["value"], "ms_per_step": result["ms_per_step"],
                            "passes_per_step": passes, "exact": "bit-exact with program order",
                            "roofline": result["roofline"]}
-    if want_tiled and "reorder" in legs:
-        ro = reorder_leg(sim, ops, n, args, per_launch, alg_bytes)
-        result["reorder"] = ro
+    for leg, comp in (("reorder", False), ("compiled", True)):
+        if not (want_tiled and leg in legs):
+            continue
+        ro = reorder_leg(sim, ops, n, args, per_launch, alg_bytes, compiled=comp)
+        result[leg] = ro
         if ro["within_tolerance"] and ro["gates_per_s"] > result["value"]:
             result.update({"value": ro["gates_per_s"], "ms_per_step": ro["ms_per_step"],
                            "roofline": ro["roofline"]})
-            config.update({"run_mode": "reorder", "passes_per_step": ro["passes_per_step"],
+            config.update({"run_mode": leg, "passes_per_step": ro["passes_per_step"],
                            "gates_per_pass": nops / ro["passes_per_step"]})
             result["hbm_TBps"] = per_launch * ro["passes_per_step"] / (ro["ms_per_step"] * 1e-3) / 1e12
             result["one_pass_per_gate_equivalent_TBps"] = alg_bytes / (ro["ms_per_step"] * 1e-3) / 1e12
-    elif pg is not None:
+    if "value" not in result and pg is not None:
         result.update({"value": pg["gates_per_s"], "ms_per_step": pg["ms_per_step"]})
         config["run_mode"] = "per_gate"
         result["roofline"] = pg["roofline"]

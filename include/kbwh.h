@@ -75,6 +75,12 @@ extern "C" {
                                 Z/P/RZ gates on one qubit are first fused across the gates diagonal on that qubit (phase
                                 products).  A Block state keeps the reference's map and swap count (decided in program
                                 order); its swaps are ordered with the gates */
+#define KBWH_RUN_COMPILED 5u /* KBWH_RUN_REORDER's passes, each run by a kernel compiled for that pass (gate kinds,
+                                tile positions and controls fixed in its code; angles read from the gate records):
+                                the first run of a pass structure compiles its kernels at run time (hiprtc, in
+                                parallel host threads) and later runs reuse them for the process; amplitudes equal to
+                                KBWH_RUN_REORDER's to rounding.  A pass that does not compile, or of more than 256
+                                gates, runs the generic kernel; a Block state runs as KBWH_RUN_REORDER */
 
 /* One circuit operation (24 bytes).  `angle` is used unless `param` >= 0
  * and a parameter vector is supplied, in which case the angle is
@@ -120,6 +126,19 @@ int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const doub
  * most non-diagonal targets outside qubits 0..2 per tiled pass, 1..64
  * (default 64: as many as the tile's free positions, 8 for 11-qubit tiles). */
 int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_targets);
+
+/* Pass-specialised kernels of KBWH_RUN_COMPILED, for the process (no
+ * counterpart in the reference): kernels compiled, passes that found theirs
+ * already compiled, compile failures, passes run by the generic kernel and
+ * the wall time spent compiling.  Any pointer may be NULL. */
+int32_t kbwh_compiled_stats(uint64_t *kernels_compiled, uint64_t *cache_hits, uint64_t *failures,
+                            uint64_t *generic_passes, double *compile_seconds);
+/* The gfx950 code object of the specialised kernel of pass `pass` of the
+ * KBWH_RUN_COMPILED plan of ops on a flat state of num_qubits (11..40)
+ * qubits, compiled on the host (no device needed; no counterpart in the
+ * reference).  With code NULL, *size receives the size in bytes. */
+int32_t kbwh_compiled_pass_code(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits, uint32_t pass,
+                                void *code, uint64_t *size);
 
 /* The pass plan of KBWH_RUN_REORDER for a flat state of num_qubits qubits,
  * on the host only (no device needed; no counterpart in the reference):

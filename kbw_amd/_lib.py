@@ -23,7 +23,7 @@ X, Y, Z, H, P, RX, RY, RZ = range(8)
 GATE_NAMES = {"x": X, "y": Y, "z": Z, "h": H, "p": P, "rx": RX, "ry": RY, "rz": RZ}
 PAULI_X, PAULI_Y, PAULI_Z = 1, 2, 3
 FLAG_BLOCK, FLAG_QUEUE = 1, 2
-RUN_PER_GATE, RUN_TILED, RUN_FUSED, RUN_AUTO, RUN_REORDER = 0, 1, 2, 3, 4
+RUN_PER_GATE, RUN_TILED, RUN_FUSED, RUN_AUTO, RUN_REORDER, RUN_COMPILED = 0, 1, 2, 3, 4, 5
 
 # kbwh_op, 24 bytes
 OP_DTYPE = np.dtype([("kind", "u1"), ("target", "u1"), ("reserved", "<u2"), ("param", "<i4"),
@@ -74,6 +74,8 @@ SIGNATURES = {
     "kbwh_batch_new": [_p, _u32, _p, _p, _p, _p, _p, _u64, C.POINTER(_vp)],
     "kbwh_batch_plan": [_p, _u32, _p, _p, _p, _p, _p, _u64, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)],
     "kbwh_reorder_plan": [_p, _u64, _u32, _u32, _p, _p, C.POINTER(_u64)],
+    "kbwh_compiled_stats": [C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_f64)],
+    "kbwh_compiled_pass_code": [_p, _u64, _u32, _u32, _p, C.POINTER(_u64)],
     "kbwh_batch_run": [_vp, _p, _p],
     "kbwh_batch_use_graph": [_vp, _i32],
     "kbwh_batch_state": [_vp, _u32, _u64, _p],

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libkbwh.so")
 BUILD_DIR = os.path.join(ROOT, "_build", "kbwh")
 
-HIP_SOURCES = ["kernels.hip", "state.hip", "observables.hip", "tiled.hip", "adjoint.hip", "batch.hip", "fused.hip", "autotune.hip", "live.hip", "adjoint_tiled.hip", "tiled_fma.hip", "tiled_pass.hip"]
+HIP_SOURCES = ["kernels.hip", "state.hip", "observables.hip", "tiled.hip", "adjoint.hip", "batch.hip", "fused.hip", "autotune.hip", "live.hip", "adjoint_tiled.hip", "tiled_fma.hip", "tiled_pass.hip", "tile_jit.hip"]
 CXX_SOURCES = ["coef.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -67,6 +67,10 @@ def build(jobs=8, verbose=False):
         obj = os.path.join(BUILD_DIR, src.replace(".cpp", ".o"))
         objs.append(obj)
         procs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    # the device tile header as a string, for the kernels compiled at run time
+    emb = embed_header(os.path.join(CSRC, "tile_dev.h"), "kTileDevSource", os.path.join(BUILD_DIR, "tile_dev_src.cpp"))
+    objs.append(emb[:-4] + ".o")
+    procs.append(["g++", *CXX_FLAGS, "-c", emb, "-o", objs[-1]])
     running = []
     for cmd in procs:
         running.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
@@ -75,9 +79,21 @@ def build(jobs=8, verbose=False):
     while running:
         _wait(running.pop(0), verbose)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"])
     os.replace(tmp, LIB)
     return LIB
+
+
+def embed_header(path, symbol, out):
+    """Writes a C++ source defining kbwh::<symbol> as the text of `path`."""
+    text = open(path).read()
+    delim = "KBWHSRC"
+    assert ")" + delim not in text
+    with open(out, "w") as f:
+        f.write(f'namespace kbwh {{\nextern const char {symbol}[];\nconst char {symbol}[] = R"{delim}(')
+        f.write(text)
+        f.write(f'){delim}";\n}}  // namespace kbwh\n')
+    return out
 
 
 def _wait(item, verbose):

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/kbwh.h"
@@ -109,9 +110,17 @@ hipError_t launch_tile_exact(double2 *amp, const void *spec, const void *store, 
 // (parameter sets laid out one after another, see run_sets_forward).
 hipError_t launch_tile_contracted(double2 *amp, const void *spec, const void *gates, uint32_t num_gates,
                                   uint32_t ntiles, hipStream_t stream, uint32_t set_shift = 32);
-// reorder: dependency-ordered passes (KBWH_RUN_REORDER; flat states only)
+// reorder: dependency-ordered passes (KBWH_RUN_REORDER; flat states only);
+// compiled: their pass-specialised kernels (KBWH_RUN_COMPILED)
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
-                     const double *params, bool reorder = false);
+                     const double *params, bool reorder = false, bool compiled = false);
+// Pass-specialised kernels (tile_jit.hip): compiles and loads on `device` the
+// kernel of each pass {host tile::TGate records, count} (cached by the
+// pass's structure); fns[i] is null where pass i runs the generic kernel.
+int32_t jit_prepare(int device, const std::vector<std::pair<const void *, uint32_t>> &passes,
+                    std::vector<void *> &fns);
+hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
+                      hipStream_t stream);
 void free_adjoint(kbwh_state *s);
 // Tiled reverse sweep of the adjoint gradient (n >= 11): slots[j] receives the
 // term of the j-th parameterised op; partial holds kAdjTiledScratch doubles.

@@ -301,6 +301,7 @@ int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const doub
     if (mode == KBWH_RUN_AUTO) mode = s->auto_mode;
     if (mode == KBWH_RUN_TILED) return run_tiled(s, ops, num_ops, params);
     if (mode == KBWH_RUN_REORDER) return run_tiled_on(s, s->amp, ops, num_ops, params, true);
+    if (mode == KBWH_RUN_COMPILED) return run_tiled_on(s, s->amp, ops, num_ops, params, true, true);
     if (mode == KBWH_RUN_FUSED) return run_fused(s, s->amp, ops, num_ops, params, (int)s->fusion_width);
     if (mode != KBWH_RUN_PER_GATE) return fail(KBWH_ERR_INVALID_ARGUMENT, "unknown run mode");
     s->last_passes = num_ops;

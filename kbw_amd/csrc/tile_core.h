@@ -14,517 +14,11 @@
 #include <vector>
 
 #include "state.h"
-
-// Exact gate arithmetic rounds every product and sum on its own, as the
-// reference does.  A unit that defines KBWH_TILE_CONTRACT (the dependency-
-// ordered runs, equal to the reference only to rounding anyway) lets the
-// compiler fuse a product into the following add.
-#ifdef KBWH_TILE_CONTRACT
-#pragma clang fp contract(fast)
-#else
-#pragma clang fp contract(off)
-#endif
+#include "tile_dev.h"
 
 namespace kbwh {
 namespace tile {
-// Internal linkage: a translation unit may be built with its own tile width
-// (KBWH_TILE_BITS), and no tile type crosses translation units.
 namespace {
-
-// Tile shape, measured on MI355X for the 30-qubit brickwork circuit: 2^11
-// amplitudes per tile (8 per thread), one tile per workgroup and 4 waves per
-// SIMD beat 2^12-amplitude tiles and an LDS-staged persistent pipeline.
-#ifndef KBWH_TILE_BITS
-#define KBWH_TILE_BITS 11
-#endif
-#ifndef KBWH_TILE_WAVES
-#define KBWH_TILE_WAVES 4
-#endif
-#ifndef KBWH_TILE_TBITS
-#define KBWH_TILE_TBITS 8
-#endif
-constexpr int TL = KBWH_TILE_BITS;   // tile qubits
-constexpr int TB = KBWH_TILE_TBITS;  // thread bits: 6 lane bits, then TB - 6 wave bits
-constexpr int TT = 1 << TB;          // threads per tile
-constexpr int TR = 1 << (TL - TB);   // amplitudes per thread (register bits = TL - TB)
-static_assert(TB >= 6 && TL - TB >= 1 && TL - TB <= 4, "tile geometry");
-// register bits that gate targets and controls may occupy (a unit that keeps
-// the top register bit for its own use lowers this)
-#ifndef KBWH_TILE_GATE_RBITS
-#define KBWH_TILE_GATE_RBITS (KBWH_TILE_BITS - KBWH_TILE_TBITS)
-#endif
-constexpr int TGR = KBWH_TILE_GATE_RBITS;
-static_assert(TGR >= 1 && TGR <= TL - TB, "gate register bits");
-constexpr uint32_t kOutside = 255;
-#ifndef KBWH_TILE_CHUNK
-#define KBWH_TILE_CHUNK 4
-#endif
-constexpr int kChunk = KBWH_TILE_CHUNK < TR ? KBWH_TILE_CHUNK : TR;  // register slots per LDS exchange round
-
-struct TGate {
-    uint32_t kind;
-    uint32_t tpos;  // tile position of the target, kOutside if the target is not in Q
-    uint32_t cm;    // controls inside the tile, as tile positions
-    uint32_t pad;
-    uint64_t ocm;   // controls outside the tile, as global bits
-    uint64_t otb;   // the target's global bit (diagonal gates with target outside Q)
-    // pair gates: out coefficients (uniform part u, part times the half's sign s)
-    double au, as, bu, bs, cru, crs, ciu, cis, d;
-    // diagonal gates: the phases of the |0> and |1> halves
-    double ph0x, ph0y, ph1x, ph1y;
-    // diagonal gates: the thread bits and the register-slot bits that must be
-    // 1 (Z and P: target and controls; RZ: controls), set by make_tgate
-    uint32_t dlo, dhi;
-};
-static_assert(sizeof(TGate) == 144, "TGate layout");
-
-struct TileSpec {
-    uint64_t qbit[TL];  // qbit[b] = 1 << Q[b]
-    uint64_t qlo[TL];   // (1 << p) - 1 for the positions p of Q in ascending order
-};
-
-enum { KX = 0, KY = 1, KZ = 2, KH = 3, KP = 4, KRX = 5, KRY = 6, KRZ = 7 };
-
-// Amplitudes in the tile kernel are clang vector values (selects on them are
-// value selects; HIP's double2 class would turn a ?: into a select between
-// addresses and push the register array out to scratch memory).
-typedef double dv2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ dv2 mk2(double x, double y) {
-    dv2 r;
-    r.x = x;
-    r.y = y;
-    return r;
-}
-
-__device__ __forceinline__ dv2 cmul(dv2 a, dv2 b) {
-    dv2 r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.y * b.x + a.x * b.y;
-    return r;
-}
-
-// m = 0 or the sign bit: flips the sign of a double / of both parts when set
-__device__ __forceinline__ double flip_sign_d(double x, uint32_t m) {
-    return __longlong_as_double(__double_as_longlong(x) ^ (long long)((unsigned long long)m << 32));
-}
-__device__ __forceinline__ dv2 flip_sign(dv2 x, uint32_t m) {
-    return mk2(flip_sign_d(x.x, m), flip_sign_d(x.y, m));
-}
-
-__device__ __forceinline__ dv2 shfl_xor2(dv2 v, int mask) {
-    return mk2(__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64));
-}
-
-// Every pair closure of the reference (dense_v2.rs:166-263), written for one
-// output amplitude from its own value m and its partner's value o:
-//     out.re = ((A m.re + B o.re) + Cr o.im) * D
-//     out.im = ((A m.im + B o.im) + Ci o.re) * D
-// with coefficients that depend on the half (sigma = +1 for the |1> amplitude,
-// -1 for the |0> one):
-//     X:  A=0     B=1        Cr=0       Ci=0      D=1
-//     Y:  A=0     B=0        Cr=-sigma  Ci=sigma  D=1
-//     H:  A=-sigma B=1       Cr=0       Ci=0      D=1/sqrt2
-//     RX: A=c     B=0        Cr=s       Ci=-s     D=1
-//     RY: A=c     B=sigma*s  Cr=0       Ci=0      D=1
-// Adding a zero product, multiplying by 1 and taking a product by +-1 are
-// exact, and the order of two addends does not change a rounded sum, so each
-// output equals the reference's closure in value (the sign of a zero aside).
-struct PairCoef {
-    double A, B, Cr, Ci, D;
-};
-
-__device__ __forceinline__ PairCoef pair_coef(const TGate &G, double sigma) {
-    PairCoef k;
-    k.A = G.au + G.as * sigma;
-    k.B = G.bu + G.bs * sigma;
-    k.Cr = G.cru + G.crs * sigma;
-    k.Ci = G.ciu + G.cis * sigma;
-    k.D = G.d;
-    return k;
-}
-
-__device__ __forceinline__ dv2 pair_out(dv2 m, dv2 o, const PairCoef &k) {
-    return mk2(((k.A * m.x + k.B * o.x) + k.Cr * o.y) * k.D, ((k.A * m.y + k.B * o.y) + k.Ci * o.x) * k.D);
-}
-
-// Uncontrolled gates take shorter forms, each equal in value to the generic
-// form above (and to the reference's closure):
-//     X:  out = o                                   (exchange)
-//     Y:  |0>: (o.im, -o.re), |1>: (-o.im, o.re)    (exchange and sign flips)
-//     H:  |0>: (m + o) k,     |1>: (o - m) k         (k = 1/sqrt2)
-//     RX: (c m.re + s o.im, c m.im - s o.re)
-//     RY: c m + sigma s o
-// RY (PK_TWO) is out = A m + B o and RX (PK_RX) out = A m + Cr q with
-// q = (o.im, -o.re), with no product by D = 1; a path each, so no amplitude
-// picks its partner's form at run time.  Controlled gates other than X use
-// the generic form (PK_GEN).  Every path is chosen by uniform two-way tests on
-// the gate record, never by a multi-way branch (see apply_pair_t).
-enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2, PK_Y = 3, PK_H = 4, PK_RX = 5 };
-
-// Control test for the amplitude in register slot r of thread tid, whose tile
-// index is tid + TT r: the thread bits are tested once per gate (tok) and the
-// slot bits against the uniform part of cm, so no per-slot index is kept.
-__device__ __forceinline__ bool thread_ctrl_ok(uint32_t tid, uint32_t cm) {
-    const uint32_t lo = cm & (uint32_t)(TT - 1);
-    return (tid & lo) == lo;
-}
-__device__ __forceinline__ bool slot_ctrl_ok(bool tok, int r, uint32_t cm) {
-    const uint32_t hi = cm >> TB;
-    return tok && (((uint32_t)r & hi) == hi);
-}
-
-template <int PK>
-__device__ __forceinline__ dv2 pair_val(dv2 m, dv2 o, bool one, const TGate &G) {
-    if constexpr (PK == PK_X) {
-        return o;
-    } else if constexpr (PK == PK_Y) {
-        return one ? mk2(-o.y, o.x) : mk2(o.y, -o.x);
-    } else if constexpr (PK == PK_H) {
-        const double sx = one ? o.x - m.x : m.x + o.x;
-        const double sy = one ? o.y - m.y : m.y + o.y;
-        return mk2(sx * G.d, sy * G.d);
-    } else if constexpr (PK == PK_TWO) {
-        // RY's record has as = bu = 0 (set_coefficients): A = au + as sigma is
-        // au and B = bu + bs sigma is sigma bs, exactly
-        const double B = one ? G.bs : -G.bs;
-        return mk2(G.au * m.x + B * o.x, G.au * m.y + B * o.y);
-    } else if constexpr (PK == PK_RX) {
-        // RX's record has as = crs = 0: A = au and Cr = cru for both halves
-        return mk2(G.au * m.x + G.cru * o.y, G.au * m.y + G.cru * -o.x);
-    } else {
-        return pair_out(m, o, pair_coef(G, one ? 1.0 : -1.0));
-    }
-}
-
-// pair_val where the half varies across lanes: sg is the sign bit when the
-// amplitude is in the |1> half.  H and Y take their half from sign flips
-// instead of computing both forms and selecting: -m + o is o - m exactly, and
-// (-o.y, o.x) / (o.y, -o.x) are sign flips of the partner.
-template <int PK>
-__device__ __forceinline__ dv2 pair_val_lane(dv2 m, dv2 o, bool one, uint32_t sg, const TGate &G) {
-    if constexpr (PK == PK_Y) {
-        return mk2(flip_sign_d(o.y, sg), flip_sign_d(o.x, sg ^ 0x80000000u));
-    } else if constexpr (PK == PK_H) {
-        return mk2((flip_sign_d(m.x, sg) + o.x) * G.d, (flip_sign_d(m.y, sg) + o.y) * G.d);
-    } else if constexpr (PK == PK_TWO) {
-        const double B = flip_sign_d(G.bs, sg ^ 0x80000000u);
-        return mk2(G.au * m.x + B * o.x, G.au * m.y + B * o.y);
-    } else {
-        return pair_val<PK>(m, o, one, G);
-    }
-}
-
-// Target on register bit RB: both amplitudes of a pair live in one thread.
-// CTRL: some control lies inside the tile, so only the pairs whose index
-// matches cm change.
-template <int RB, int PK, bool CTRL>
-__device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, const TGate &G) {
-    // An exchange has no instructions of its own: without this marker the
-    // optimizer folds the register-bit variants of X into selects between
-    // every candidate partner.
-    if constexpr (PK == PK_X) asm volatile("");
-    const uint32_t cm = G.cm;
-    const bool tok = thread_ctrl_ok(tid, cm);
-#pragma unroll
-    for (int r = 0; r < TR; ++r) {
-        if ((r >> RB) & 1) continue;
-        const int r1 = r | (1 << RB);
-        const dv2 t0 = v[r], t1 = v[r1];
-        const dv2 o0 = pair_val<PK>(t0, t1, false, G);
-        const dv2 o1 = pair_val<PK>(t1, t0, true, G);
-        if constexpr (CTRL) {
-            const bool ok = slot_ctrl_ok(tok, r, cm);
-            v[r] = ok ? o0 : t0;
-            v[r1] = ok ? o1 : t1;
-        } else {
-            v[r] = o0;
-            v[r1] = o1;
-        }
-    }
-}
-
-// Lane exchanges.  Lane layouts pinned on gfx950:
-//   xor 1, 2:  DPP quad_perm 0xB1 / 0x4E gives lane i the value of lane i^1 / i^2
-//   xor 8:     DPP row_ror:8 (rotate each 16-lane row by 8) gives lane i^8
-//   xor 4:     row_half_mirror (lane i^7 within each 8-lane half row), then
-//              quad_perm [3,2,1,0] (lane i^3): two moves, no select
-//   xor 16/32: ds_bpermute.  v_permlane16/32_swap of a register with itself
-//              (then a select on the lane bit) was measured slower: RY on
-//              lane bit 4 or 5 took 510 / 655 us per 30-qubit pass against
-//              338 / 399 us.
-enum { LX_Q1 = 0, LX_Q2 = 1, LX_R8 = 2, LX_R4 = 3, LX_SHFL = 4 };
-
-template <int LX>
-__device__ __forceinline__ unsigned lane_xor_u(unsigned x) {
-    if constexpr (LX == LX_Q1) {
-        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
-    } else if constexpr (LX == LX_Q2) {
-        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
-    } else if constexpr (LX == LX_R8) {
-        return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, false);
-    } else {
-        const int m7 = __builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, false);
-        return (unsigned)__builtin_amdgcn_mov_dpp(m7, 0x1B, 0xF, 0xF, false);
-    }
-}
-
-template <int LX>
-__device__ __forceinline__ double lane_xor_d(double x, int lm) {
-    if constexpr (LX == LX_SHFL) {
-        return __shfl_xor(x, lm, 64);
-    } else {
-        const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-        const unsigned lo = lane_xor_u<LX>((unsigned)b);
-        const unsigned hi = lane_xor_u<LX>((unsigned)(b >> 32));
-        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-    }
-}
-
-template <int PK, bool CTRL, int LX>
-__device__ __forceinline__ void lane_pair(dv2 (&v)[TR], uint32_t tid, uint32_t tp, const TGate &G) {
-    const uint32_t cm = G.cm;
-    const bool tok = thread_ctrl_ok(tid, cm);
-    const bool one = ((tid >> tp) & 1u) != 0;
-    const uint32_t sg = one ? 0x80000000u : 0u;
-    const int lm = 1 << tp;
-#pragma unroll
-    for (int r = 0; r < TR; ++r) {
-        const dv2 o = mk2(lane_xor_d<LX>(v[r].x, lm), lane_xor_d<LX>(v[r].y, lm));
-        const dv2 out = pair_val_lane<PK>(v[r], o, one, sg, G);
-        if constexpr (CTRL) {
-            v[r] = slot_ctrl_ok(tok, r, cm) ? out : v[r];
-        } else {
-            v[r] = out;
-        }
-    }
-}
-
-template <int PK, bool CTRL>
-__device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, uint32_t tp,
-                                            const TGate &G) {
-    const uint32_t cm = G.cm;
-    const bool tok = thread_ctrl_ok(tid, cm);
-    const bool one = ((tid >> tp) & 1u) != 0;
-    if (tp < 6) {
-        // dispatched on the position's bits (no multi-way branch; see apply_pair_t)
-        if (tp & 4u) {
-            lane_pair<PK, CTRL, LX_SHFL>(v, tid, tp, G);
-        } else if (tp & 2u) {
-            if (tp & 1u)
-                lane_pair<PK, CTRL, LX_R8>(v, tid, tp, G);
-            else
-                lane_pair<PK, CTRL, LX_R4>(v, tid, tp, G);
-        } else if (tp & 1u) {
-            lane_pair<PK, CTRL, LX_Q2>(v, tid, tp, G);
-        } else {
-            lane_pair<PK, CTRL, LX_Q1>(v, tid, tp, G);
-        }
-        return;
-    }
-    // wave bits: through LDS, kChunk register slots at a time
-    const uint32_t pt = tid ^ (1u << tp);
-#pragma unroll
-    for (int q = 0; q < TR; q += kChunk) {
-        __syncthreads();  // earlier exchange reads are done
-#pragma unroll
-        for (int r = 0; r < kChunk; ++r) lds[tid + (r << TB)] = v[q + r];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kChunk; ++r) {
-            const dv2 o = lds[pt + (r << TB)];
-            const dv2 out = pair_val_lane<PK>(v[q + r], o, one, one ? 0x80000000u : 0u, G);
-            if constexpr (CTRL) {
-                v[q + r] = slot_ctrl_ok(tok, q + r, cm) ? out : v[q + r];
-            } else {
-                v[q + r] = out;
-            }
-        }
-    }
-}
-
-template <int PK, bool CTRL>
-__device__ __forceinline__ void apply_pair_t(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
-    const uint32_t tp = G.tpos;
-    if (tp >= (uint32_t)TB) {
-        // Register bit rb = tp - TB, dispatched on its bits.  A `switch (tp)`
-        // here was lowered by the gfx950 backend into a compare tree whose
-        // default case had two predecessors, and the structurized code never
-        // ran that case (the last register bit's gates were dropped).
-        const uint32_t rb = tp - TB;
-        if constexpr (TL - TB > 2) {
-            if (rb & 2u) {
-                if constexpr (TL - TB > 3) {
-                    if (rb & 1u)
-                        reg_pair<3, PK, CTRL>(v, tid, G);
-                    else
-                        reg_pair<2, PK, CTRL>(v, tid, G);
-                } else {
-                    reg_pair<2, PK, CTRL>(v, tid, G);
-                }
-                return;
-            }
-        }
-        if constexpr (TL - TB > 1) {
-            if (rb & 1u) {
-                reg_pair<1, PK, CTRL>(v, tid, G);
-                return;
-            }
-        }
-        reg_pair<0, PK, CTRL>(v, tid, G);
-        return;
-    }
-    thread_pair<PK, CTRL>(v, lds, tid, tp, G);
-}
-
-__device__ __forceinline__ void apply_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
-    if (G.kind == KX) {
-        if (G.cm == 0)
-            apply_pair_t<PK_X, false>(v, lds, tid, G);
-        else
-            apply_pair_t<PK_X, true>(v, lds, tid, G);
-        return;
-    }
-    if (G.cm != 0)
-        apply_pair_t<PK_GEN, true>(v, lds, tid, G);
-    else if (G.kind == KY)
-        apply_pair_t<PK_Y, false>(v, lds, tid, G);
-    else if (G.kind == KH)
-        apply_pair_t<PK_H, false>(v, lds, tid, G);
-    else if (G.kind == KRX)
-        apply_pair_t<PK_RX, false>(v, lds, tid, G);
-    else
-        apply_pair_t<PK_TWO, false>(v, lds, tid, G);
-}
-
-// Z, P and RZ: each amplitude is multiplied by the phase of its half
-// (dense_v2.rs:190-191, 215-216, 271-272); Z and P leave the |0> half alone.
-// Z's product by -1 is the negation in value, a flip of the two sign bits.
-//
-// Z and P change the amplitudes whose target and control bits are all 1.
-// Those bits split into thread bits (a sign mask or a phase per thread),
-// register bits (a mask MH of register slots, dispatched once per gate to code
-// that touches exactly the slots r with r & MH == MH) and bits outside the
-// tile (fixed per tile: the gate is skipped where one is 0).  A thread that is
-// not selected multiplies by exactly 1 + 0i, which returns the value (a zero
-// may change sign).  RZ changes every slot whose controls hold: the phase
-// comes per thread (target on a thread bit) or per slot (target on a register
-// bit, or outside the tile).  No per-amplitude branch, no per-slot branch.
-enum { DK_Z = 0, DK_P = 1, DK_RZ = 2 };
-
-template <int N>
-struct IC {
-    static constexpr int value = N;
-};
-
-// f(IC<m>()) for the runtime mask m of NB register bits, by tests of its bits
-// (no multi-way branch; see apply_pair_t)
-template <int NB, int ACC, class F>
-__device__ __forceinline__ void with_slot_mask(uint32_t m, F &f) {
-    if constexpr (NB == 0) {
-        f(IC<ACC>());
-    } else {
-        if (m & (1u << (NB - 1)))
-            with_slot_mask<NB - 1, ACC | (1 << (NB - 1))>(m, f);
-        else
-            with_slot_mask<NB - 1, ACC>(m, f);
-    }
-}
-
-template <int DK>
-__device__ __forceinline__ void apply_diag_t(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
-    const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
-    const uint32_t cm = G.cm, tp = G.tpos;
-    const uint32_t rb = tp - (uint32_t)TB;
-    const bool reg_t = rb < 8u;         // target on a register bit
-    const bool out_t = tp == kOutside;  // target outside the tile: its bit is fixed per tile
-    const bool thr_t = !reg_t && !out_t;
-    const uint32_t clo = G.dlo, chi = G.dhi;
-    if constexpr (DK == DK_Z || DK == DK_P) {
-        // dlo / dhi hold target and controls; a target outside the tile is
-        // among the record's outside controls, tested before the gate
-        const uint32_t mlo = clo, mhi = chi;
-        const bool sel = (tid & mlo) == mlo;
-        if constexpr (DK == DK_Z) {
-            const uint32_t m = sel ? 0x80000000u : 0u;
-            auto body = [&](auto mh) {
-                constexpr int MH = decltype(mh)::value;
-#pragma unroll
-                for (int r = 0; r < TR; ++r)
-                    if ((r & MH) == MH) v[r] = flip_sign(v[r], m);
-            };
-            with_slot_mask<TGR, 0>(mhi, body);
-        } else {
-            const dv2 f = sel ? ph1 : mk2(1.0, 0.0);
-            auto body = [&](auto mh) {
-                constexpr int MH = decltype(mh)::value;
-#pragma unroll
-                for (int r = 0; r < TR; ++r)
-                    if ((r & MH) == MH) v[r] = cmul(v[r], f);
-            };
-            with_slot_mask<TGR, 0>(mhi, body);
-        }
-    } else {
-        const bool ok = (tid & clo) == clo;
-        if (!reg_t) {
-            // one phase per thread (target on a thread bit) or per tile (outside)
-            const bool one = thr_t ? ((tid >> (tp & 31u)) & 1u) != 0 : (base & G.otb) != 0;
-            const dv2 f = ok ? (one ? ph1 : ph0) : mk2(1.0, 0.0);
-            auto body = [&](auto mh) {
-                constexpr int MH = decltype(mh)::value;
-#pragma unroll
-                for (int r = 0; r < TR; ++r)
-                    if ((r & MH) == MH) v[r] = cmul(v[r], f);
-            };
-            with_slot_mask<TGR, 0>(chi, body);
-        } else if (cm == 0) {
-            // target on register bit RB: every slot, each with a fixed phase
-            auto body = [&](auto tb) {
-                constexpr int TM = decltype(tb)::value;
-                if constexpr (TM != 0 && (TM & (TM - 1)) == 0) {
-#pragma unroll
-                    for (int r = 0; r < TR; ++r) v[r] = cmul(v[r], (r & TM) ? ph1 : ph0);
-                }
-            };
-            with_slot_mask<TGR, 0>(1u << rb, body);
-        } else {
-            // controlled, target on a register bit: the phase of slot r is
-            // picked by uniform selects; slots and threads left out multiply by 1
-#pragma unroll
-            for (int r = 0; r < TR; ++r) {
-                const dv2 ph = ((uint32_t)r & chi) == chi ? (((r >> rb) & 1u) ? ph1 : ph0) : mk2(1.0, 0.0);
-                v[r] = cmul(v[r], ok ? ph : mk2(1.0, 0.0));
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ void apply_diag(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
-    if (G.kind == KZ)
-        apply_diag_t<DK_Z>(v, base, tid, G);
-    else if (G.kind == KP)
-        apply_diag_t<DK_P>(v, base, tid, G);
-    else
-        apply_diag_t<DK_RZ>(v, base, tid, G);
-}
-
-// The tile number spread over the bits outside Q (a 0 inserted at each position of Q).
-__device__ __forceinline__ uint64_t tile_base(uint64_t k, const TileSpec &ts) {
-#pragma unroll
-    for (int j = 0; j < TL; ++j) {
-        const uint64_t lo = k & ts.qlo[j];
-        k = ((k ^ lo) << 1) | lo;
-    }
-    return k;
-}
-
-__device__ __forceinline__ uint64_t slot_offset(uint64_t dt, int r, const TileSpec &ts) {
-    uint64_t o = dt;
-#pragma unroll
-    for (int b = 0; b < TL - TB; ++b)
-        if ((r >> b) & 1) o |= ts.qbit[TB + b];
-    return o;
-}
-
 
 inline bool is_diagonal(int kind) { return kind == KZ || kind == KP || kind == KRZ; }
 

@@ -1,0 +1,335 @@
+// Pass-specialised tile kernels (KBWH_RUN_COMPILED), compiled at run time.
+//
+// The generic pass kernel (k_tile_fma) reads each gate record and picks the
+// gate's code path by uniform tests on its kind, tile position and masks.
+// Those tests cost about 22 scalar instructions per gate, and the register
+// allocator copies the register tile where the paths join (about 16 vector
+// moves per gate), so a pass of 40+ gates is bound by instruction issue.  A
+// specialised kernel is the same load / gates / store, with each gate's
+// structural fields (kind, tile position, controls inside and outside the
+// tile, diagonal masks) written into the source as constants: the dispatch
+// folds away at compile time and the gates run as one straight line.  The
+// coefficients are still read from the gate records, so one kernel serves
+// every pass of the same structure whatever its angles.
+//
+// Kernels are compiled with hiprtc (loaded on first use; a library without it
+// runs the generic kernel) from the device tile header (tile_dev.h, embedded
+// in the library at build time), cached for the process by the pass's
+// structure, and loaded once per device.  The passes a run needs are compiled
+// in parallel host threads.
+#include <dlfcn.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <atomic>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include <hip/hiprtc.h>
+
+#define KBWH_TILE_CONTRACT 1
+#include "state.h"
+#include "tile_core.h"
+
+namespace kbwh {
+
+extern const char kTileDevSource[];  // tile_dev.h, embedded by the build
+
+namespace {
+
+using namespace tile;
+
+// the hiprtc entry points, resolved from libhiprtc at first use
+struct Rtc {
+    bool ok = false;
+    std::string why;
+    decltype(&hiprtcCreateProgram) create = nullptr;
+    decltype(&hiprtcCompileProgram) compile = nullptr;
+    decltype(&hiprtcDestroyProgram) destroy = nullptr;
+    decltype(&hiprtcGetCodeSize) code_size = nullptr;
+    decltype(&hiprtcGetCode) code = nullptr;
+    decltype(&hiprtcGetProgramLogSize) log_size = nullptr;
+    decltype(&hiprtcGetProgramLog) log = nullptr;
+};
+
+const Rtc &rtc() {
+    static Rtc r;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        void *h = dlopen("libhiprtc.so.7", RTLD_NOW | RTLD_LOCAL);
+        if (!h) h = dlopen("libhiprtc.so", RTLD_NOW | RTLD_LOCAL);
+        if (!h) {
+            r.why = "libhiprtc not found";
+            return;
+        }
+        r.create = (decltype(r.create))dlsym(h, "hiprtcCreateProgram");
+        r.compile = (decltype(r.compile))dlsym(h, "hiprtcCompileProgram");
+        r.destroy = (decltype(r.destroy))dlsym(h, "hiprtcDestroyProgram");
+        r.code_size = (decltype(r.code_size))dlsym(h, "hiprtcGetCodeSize");
+        r.code = (decltype(r.code))dlsym(h, "hiprtcGetCode");
+        r.log_size = (decltype(r.log_size))dlsym(h, "hiprtcGetProgramLogSize");
+        r.log = (decltype(r.log))dlsym(h, "hiprtcGetProgramLog");
+        r.ok = r.create && r.compile && r.destroy && r.code_size && r.code && r.log_size && r.log;
+        if (!r.ok) r.why = "libhiprtc lacks an entry point";
+    });
+    return r;
+}
+
+// The structure of a pass: the fields of each record that select code.
+std::string pass_key(const TGate *g, uint32_t count) {
+    std::string k;
+    k.reserve(count * 40);
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint64_t f[7] = {g[i].kind, g[i].tpos, g[i].cm, g[i].ocm, g[i].otb, g[i].dlo, g[i].dhi};
+        k.append((const char *)f, sizeof f);
+    }
+    return k;
+}
+
+uint64_t fnv1a(const std::string &s) {
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) h = (h ^ c) * 1099511628211ull;
+    return h;
+}
+
+std::string pass_source(const TGate *g, uint32_t count, const std::string &name) {
+    std::string s;
+    s.reserve(1024 + count * 200);
+    s += "#define KBWH_TILE_CONTRACT 1\n#include \"tile_dev.h\"\nusing namespace kbwh::tile;\n";
+    s += "extern \"C\" __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, "
+         "KBWH_TILE_WAVES))) void ";
+    s += name;
+    s += "(dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gp, uint32_t ntiles) {\n"
+         "  __shared__ dv2 lds[TT * kChunk];\n"
+         "  const uint32_t tid = threadIdx.x, tile = blockIdx.x;\n"
+         "  if (tile >= ntiles) return;\n"
+         "  const uint64_t base = tile_base(tile, ts);\n"
+         "  const uint64_t dt = thread_offset(base, tid, ts);\n"
+         "  dv2 v[TR];\n"
+         "  tile_load(v, a, dt, ts);\n";
+    char line[320];
+    for (uint32_t i = 0; i < count; ++i) {
+        const TGate &G = g[i];
+        snprintf(line, sizeof line,
+                 "  { TGate G = gp[%u]; G.kind = %uu; G.tpos = %uu; G.cm = %uu; G.ocm = %lluull; G.otb = %lluull; "
+                 "G.dlo = %uu; G.dhi = %uu; apply_gate(v, lds, base, tid, G); }\n",
+                 i, G.kind, G.tpos, G.cm, (unsigned long long)G.ocm, (unsigned long long)G.otb, G.dlo, G.dhi);
+        s += line;
+    }
+    s += "  tile_store(v, a, dt, ts);\n}\n";
+    return s;
+}
+
+const char *const kOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                                "-Xclang", "-target-feature", "-Xclang",
+                                "-packed-fp32-ops"};
+
+// compiles one pass kernel to a code object; false with a log on failure
+bool compile_source(const std::string &src, std::vector<char> &code, std::string &log) {
+    const Rtc &r = rtc();
+    if (!r.ok) {
+        log = r.why;
+        return false;
+    }
+    hiprtcProgram p;
+    const char *hdr[] = {kTileDevSource};
+    const char *inc[] = {"tile_dev.h"};
+    if (r.create(&p, src.c_str(), "kbwh_pass.hip", 1, hdr, inc) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const hiprtcResult rc = r.compile(p, (int)(sizeof kOptions / sizeof kOptions[0]), kOptions);
+    size_t n = 0;
+    if (rc != HIPRTC_SUCCESS) {
+        if (r.log_size(p, &n) == HIPRTC_SUCCESS && n > 1) {
+            log.resize(n);
+            r.log(p, &log[0]);
+        } else {
+            log = "hiprtc compile failed";
+        }
+        r.destroy(&p);
+        return false;
+    }
+    bool ok = r.code_size(p, &n) == HIPRTC_SUCCESS && n > 0;
+    if (ok) {
+        code.resize(n);
+        ok = r.code(p, code.data()) == HIPRTC_SUCCESS;
+    }
+    r.destroy(&p);
+    if (!ok) log = "hiprtc returned no code";
+    return ok;
+}
+
+constexpr int kMaxDevices = 64;
+
+struct Kernel {
+    std::string name;
+    std::vector<char> code;
+    bool failed = false;
+    std::string log;
+    hipModule_t mod[kMaxDevices] = {};
+    hipFunction_t fn[kMaxDevices] = {};
+};
+
+struct Cache {
+    std::mutex mu;
+    std::map<std::string, std::shared_ptr<Kernel>> kernels;
+    std::atomic<uint64_t> compiled{0}, hits{0}, failures{0}, generic{0};
+    std::atomic<uint64_t> compile_us{0};
+};
+
+Cache &cache() {
+    static Cache *c = new Cache;  // never destroyed: modules stay loaded until the process ends
+    return *c;
+}
+
+}  // namespace
+
+// Passes longer than this run the generic kernel: a longer straight line
+// compiles slowly and gains little once a pass is bound by its gate work.
+constexpr uint32_t kMaxSpecialisedGates = 256;
+
+// Compiles (in parallel) every pass of the list not compiled yet, loads the
+// kernels on `device` and returns one function per pass (null: run the
+// generic kernel for it).  passes[i] = {host records, count}.
+int32_t jit_prepare(int device, const std::vector<std::pair<const void *, uint32_t>> &passes,
+                    std::vector<void *> &fns) {
+    Cache &C = cache();
+    fns.assign(passes.size(), nullptr);
+    if (device < 0 || device >= kMaxDevices) return KBWH_OK;
+    // one prepare at a time: a kernel in the cache is compiled (or failed)
+    std::lock_guard<std::mutex> lk(C.mu);
+    std::vector<std::shared_ptr<Kernel>> ks(passes.size());
+    std::vector<std::pair<std::shared_ptr<Kernel>, std::string>> todo;
+    {
+        for (size_t i = 0; i < passes.size(); ++i) {
+            const TGate *g = (const TGate *)passes[i].first;
+            const uint32_t cnt = passes[i].second;
+            if (cnt == 0 || cnt > kMaxSpecialisedGates) {
+                C.generic++;
+                continue;
+            }
+            std::string key = pass_key(g, cnt);
+            auto it = C.kernels.find(key);
+            if (it != C.kernels.end()) {
+                ks[i] = it->second;
+                C.hits++;
+                continue;
+            }
+            auto k = std::make_shared<Kernel>();
+            char nm[64];
+            snprintf(nm, sizeof nm, "kbwh_pass_%016llx_%u", (unsigned long long)fnv1a(key), cnt);
+            k->name = nm;
+            C.kernels.emplace(key, k);
+            ks[i] = k;
+            todo.push_back({k, pass_source(g, cnt, k->name)});
+        }
+    }
+    if (!todo.empty()) {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (size_t j; (j = next.fetch_add(1)) < todo.size();) {
+                Kernel &k = *todo[j].first;
+                k.failed = !compile_source(todo[j].second, k.code, k.log);
+            }
+        };
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = std::min<size_t>(todo.size(), std::min(16u, hw));
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < nt; ++t) th.emplace_back(work);
+        work();
+        for (auto &t : th) t.join();
+        C.compile_us += (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                            std::chrono::steady_clock::now() - t0)
+                            .count();
+        for (auto &w : todo) (w.first->failed ? C.failures : C.compiled)++;
+    }
+    // modules load into the current device: make the state's device current
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+    for (size_t i = 0; e == hipSuccess && i < passes.size(); ++i) {
+        Kernel *k = ks[i].get();
+        if (!k || k->failed) continue;
+        if (!k->fn[device]) {
+            e = hipModuleLoadData(&k->mod[device], k->code.data());
+            if (e == hipSuccess) e = hipModuleGetFunction(&k->fn[device], k->mod[device], k->name.c_str());
+            if (e != hipSuccess) break;
+        }
+        fns[i] = (void *)k->fn[device];
+    }
+    if (cur >= 0 && cur != device) {
+        const hipError_t r = hipSetDevice(cur);
+        if (e == hipSuccess) e = r;
+    }
+    if (e != hipSuccess) return hip_status(e, "loading a compiled pass kernel");
+    return KBWH_OK;
+}
+
+hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
+                      hipStream_t stream) {
+    TileSpec ts;
+    memcpy(&ts, spec, sizeof ts);
+    dv2 *a = (dv2 *)amp;
+    const TGate *g = (const TGate *)gates;
+    void *args[] = {&a, &ts, &g, &ntiles};
+    return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
+}
+
+}  // namespace kbwh
+
+extern "C" int32_t kbwh_compiled_stats(uint64_t *kernels_compiled, uint64_t *cache_hits, uint64_t *failures,
+                                       uint64_t *generic_passes, double *compile_seconds) {
+    using namespace kbwh;
+    Cache &C = cache();
+    if (kernels_compiled) *kernels_compiled = C.compiled;
+    if (cache_hits) *cache_hits = C.hits;
+    if (failures) *failures = C.failures;
+    if (generic_passes) *generic_passes = C.generic;
+    if (compile_seconds) *compile_seconds = (double)C.compile_us * 1e-6;
+    return KBWH_OK;
+}
+
+// The code object of the specialised kernel of pass `pass` of the
+// dependency-ordered plan of ops on a flat state of num_qubits qubits (host
+// only, no device).  With code = NULL, *size receives the size.
+extern "C" int32_t kbwh_compiled_pass_code(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits,
+                                           uint32_t pass, void *code, uint64_t *size) {
+    using namespace kbwh;
+    if (!ops || !size) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_qubits < TL || num_qubits > 40) return set_error(KBWH_ERR_INVALID_ARGUMENT, "11..40 qubits");
+    std::vector<PhysOp> phys(num_ops);
+    for (uint64_t i = 0; i < num_ops; ++i) {
+        const kbwh_op &op = ops[i];
+        if (op.kind > KRZ || op.target >= num_qubits || (op.ctrl_mask >> num_qubits))
+            return set_error(KBWH_ERR_INVALID_ARGUMENT, "op out of range");
+        phys[i] = PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), op.angle};
+    }
+    fuse_diagonals(phys, num_qubits);
+    std::vector<uint32_t> order, ends;
+    schedule_passes(phys, num_qubits, TL - 3, order, ends);
+    if (pass >= ends.size()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "no such pass");
+    Pass cur;
+    for (size_t k = pass ? ends[pass - 1] : 0; k < ends[pass]; ++k) add_op(cur, phys[order[k]]);
+    int pos_of[64];
+    uint32_t q_at[TL];
+    assign_positions(num_qubits, cur, pos_of, q_at);
+    std::vector<TGate> g;
+    for (const PhysOp &o : cur.ops) g.push_back(make_tgate(o, pos_of, num_qubits));
+    std::vector<char> obj;
+    std::string log;
+    if (!compile_source(pass_source(g.data(), (uint32_t)g.size(), "kbwh_pass"), obj, log))
+        return set_error(KBWH_ERR_EXECUTION_FAILED, "pass kernel did not compile: " + log.substr(0, 400));
+    if (code) {
+        if (*size < obj.size()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "buffer too small");
+        memcpy(code, obj.data(), obj.size());
+    }
+    *size = obj.size();
+    return KBWH_OK;
+}

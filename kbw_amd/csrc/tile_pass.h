@@ -43,25 +43,13 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     const uint32_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     if (set_shift < 32) gates += (size_t)(tile >> set_shift) * ng;
-    uint64_t tmask = 0;
-#pragma unroll
-    for (int b = 0; b < TB; ++b)
-        if ((tid >> b) & 1u) tmask |= ts.qbit[b];
     const uint64_t base = tile_base(tile, ts);
-    const uint64_t dt = base | tmask;
+    const uint64_t dt = thread_offset(base, tid, ts);
     dv2 v[TR];
-#pragma unroll
-    for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
+    tile_load(v, a, dt, ts);
     // ng >= 1 (the host launches only non-empty passes); a do-while leaves no
     // path around the loop, so the loaded values need no copy kept for one.
-    auto step = [&](const TGate &G) {
-        if ((base & G.ocm) == G.ocm) {  // a control outside the tile is 0 here
-            if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
-                apply_diag(v, base, tid, G);
-            else
-                apply_pair(v, lds, tid, G);
-        }
-    };
+    auto step = [&](const TGate &G) { apply_gate(v, lds, base, tid, G); };
     uint32_t i = 0;
     if constexpr (PF) {
         TGate G = gates[0];
@@ -80,26 +68,14 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
             step(G);
         } while (++gp != ge);
     }
-    // recompute the store addresses instead of keeping eight of them live
-    // through the gate loop
     if constexpr (PERM) {
         // the tile goes to addresses that other waves of the workgroup
         // loaded: every load must have returned before any store is issued
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        uint64_t sm = 0;
-#pragma unroll
-        for (int b = 0; b < TB; ++b)
-            if ((tid >> b) & 1u) sm |= st.qbit[b];
-        uint64_t ds = base | sm;
-        asm volatile("" : "+v"(ds));
-#pragma unroll
-        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, st));
+        tile_store(v, a, thread_offset(base, tid, st), st);
     } else {
-        uint64_t ds = dt;
-        asm volatile("" : "+v"(ds));
-#pragma unroll
-        for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(ds, r, ts));
+        tile_store(v, a, dt, ts);
     }
 }
 

@@ -383,6 +383,7 @@ struct PassBuilder {
     double2 *amp;
     uint32_t n, cap;
     bool contract;  // flat dependency-ordered runs: the contracted kernel copy
+    bool compiled = false;  // ... with pass-specialised kernels where they compile
     Pass cur;
     std::vector<uint32_t> ld, ph;
     bool swapped = false;
@@ -448,13 +449,25 @@ struct PassBuilder {
             if (e != hipSuccess) return hip_status(e, "tile gate buffer");
             s->tile_gates_cap = bytes;
         }
+        std::vector<void *> fns;
+        if (compiled) {
+            std::vector<std::pair<const void *, uint32_t>> lst;
+            lst.reserve(launches.size());
+            for (const Launch &L : launches)
+                lst.push_back({L.perm ? nullptr : (const void *)&gates[L.first], L.perm ? 0u : L.count});
+            const int32_t rc = jit_prepare(s->device, lst, fns);
+            if (rc) return rc;
+        }
         hipError_t e = hipMemcpyAsync(s->tile_gates, gates.data(), bytes, hipMemcpyHostToDevice, s->stream);
         if (e != hipSuccess) return hip_status(e, "tile gate upload");
         s->last_passes += launches.size();
         const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
         const TGate *dg = (const TGate *)s->tile_gates;
-        for (const Launch &L : launches) {
-            if (contract && !L.perm)
+        for (size_t i = 0; i < launches.size(); ++i) {
+            const Launch &L = launches[i];
+            if (!fns.empty() && fns[i])
+                e = jit_launch(fns[i], amp, &L.spec, dg + L.first, ntiles, s->stream);
+            else if (contract && !L.perm)
                 e = launch_tile_contracted(amp, &L.spec, dg + L.first, L.count, ntiles, s->stream);
             else
                 e = launch_tile_exact(amp, &L.spec, L.perm ? &L.store : nullptr, dg + L.first, L.count, ntiles,
@@ -649,7 +662,7 @@ int32_t run_events(kbwh_state *s, double2 *amp, const std::vector<kbwh_phys_even
 }
 
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
-                     const double *params, bool reorder) {
+                     const double *params, bool reorder, bool compiled) {
     if (s->n < TL) {
         // the whole state is smaller than one tile: one pass per gate
         if (amp == s->amp) return kbwh_run(s, ops, num_ops, params, KBWH_RUN_PER_GATE);
@@ -667,6 +680,7 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
     int32_t rc = check_ops(n, ops, num_ops);
     if (rc) return rc;
     PassBuilder pb(s, amp, reorder && !s->block);
+    pb.compiled = compiled && reorder && !s->block;
     // the map replayed in program order, exactly as gate by gate: a Block
     // state's swaps become events before the gates that need them
     auto to_events = [&](std::vector<kbwh_phys_event> &ev) {

@@ -87,6 +87,29 @@ def reorder_plan(ops, num_qubits, tile_targets=64):
     return order.astype(np.int64), list(zip(starts.tolist(), ends.tolist()))
 
 
+def compiled_stats():
+    """Pass-specialised kernels of RUN_COMPILED in this process: a dict of
+    kernels compiled, cache hits, failures, passes run by the generic kernel
+    and seconds spent compiling."""
+    v = [C.c_uint64() for _ in range(4)]
+    sec = C.c_double()
+    check(_lib.load().kbwh_compiled_stats(*[C.byref(x) for x in v], C.byref(sec)))
+    return {"compiled": v[0].value, "cache_hits": v[1].value, "failures": v[2].value,
+            "generic_passes": v[3].value, "compile_seconds": sec.value}
+
+
+def compiled_pass_code(ops, num_qubits, pass_index):
+    """The gfx950 code object (bytes) of the RUN_COMPILED kernel of one pass,
+    compiled on the host."""
+    a = ops_array(ops)
+    size = C.c_uint64()
+    lib = _lib.load()
+    check(lib.kbwh_compiled_pass_code(ptr(a), len(a), num_qubits, pass_index, None, C.byref(size)))
+    buf = C.create_string_buffer(size.value)
+    check(lib.kbwh_compiled_pass_code(ptr(a), len(a), num_qubits, pass_index, buf, C.byref(size)))
+    return buf.raw[:size.value]
+
+
 class DenseHip:
     """QuantumExecution on one MI355X: the dense c128 state lives in HBM."""
 

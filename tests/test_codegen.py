@@ -80,3 +80,32 @@ def test_tile_kernels_keep_their_register_budget(kernels, pattern, limit):
     assert found, pattern
     over = {k: n for k, n in found.items() if n > limit}
     assert not over, over
+
+
+def _notes(path):
+    notes = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "--notes", str(path)], check=True,
+                           capture_output=True, text=True).stdout
+    out = {}
+    for key in ("private_segment_fixed_size", "vgpr_count"):
+        m = re.search(r"^\s+\." + key + r":\s+(\d+)", notes, re.M)
+        out[key] = int(m.group(1)) if m else None
+    return out
+
+
+@pytest.mark.parametrize("pass_index", [0, 5, 16])
+def test_compiled_pass_kernels_keep_registers(tmp_path, pass_index):
+    """Kernels compiled at run time for single passes of the 30-qubit brickwork
+    (50, 92 and 67 gates): no scratch memory, and within the pass kernels'
+    register budget."""
+    if not os.path.exists(LIB):
+        pytest.skip("kbw_amd/libkbwh.so is not built")
+    if not _tools():
+        pytest.skip("llvm-readelf not available")
+    from kbw_amd import circuits, compiled_pass_code
+
+    ops, _ = circuits.brickwork(30, 20, seed=1)
+    co = tmp_path / "pass.co"
+    co.write_bytes(compiled_pass_code(ops, 30, pass_index))
+    meta = _notes(co)
+    assert meta["private_segment_fixed_size"] == 0, meta
+    assert meta["vgpr_count"] is not None and meta["vgpr_count"] <= 96, meta

@@ -1,0 +1,152 @@
+"""Pass-specialised kernels (KBWH_RUN_COMPILED) against the per-gate path,
+the oracle and the dependency-ordered generic kernel (KBWH_RUN_REORDER).
+
+A compiled pass runs the same gate arithmetic as the contracted generic pass
+kernel, with each gate's kind, tile position and masks fixed in its code, so
+it agrees with the per-gate path to rounding (1e-13 on a unit-norm state) and
+with KBWH_RUN_REORDER to rounding.  Each case puts every gate of one kind into
+one pass, so a kind's every tile position and control placement compiles once.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from kbw_amd import (DenseHip, RUN_COMPILED, RUN_PER_GATE, RUN_REORDER, circuits, compiled_stats)
+
+pytestmark = pytest.mark.gpu
+
+
+def random_state(rng, n):
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return st / np.linalg.norm(st)
+
+
+def run_modes(n, st, ops, modes):
+    out = []
+    for m in modes:
+        g = DenseHip(n)
+        g.set_state(st)
+        g.run(ops, mode=m)
+        out.append((g.state(), g.run_stats()[1]))
+        g.close()
+    return out
+
+
+@pytest.mark.parametrize("kind", range(8))
+def test_compiled_every_position_and_control_placement(kind):
+    """One pass of 44 gates of `kind` on a 13-qubit state: targets on all 11
+    tile qubits (lane bits 0..5, wave bits 6..7, register bits 8..10 in some
+    bijection), each uncontrolled, with a control on a lane qubit, on a tile
+    qubit 3..10 and outside the tile (qubit 12); angles covering every sign
+    and magnitude form of the half-angle coefficients."""
+    print(f"compiled kind {kind}", flush=True)
+    n = 13
+    rng = np.random.default_rng(140 + kind)
+    st = random_state(rng, n)
+    ops = circuits.OpList()
+    for q in range(11):
+        lane_c = [0] if q != 0 else [1]
+        tile_c = [3] if q != 3 else [4]
+        for ctrl, ang in zip(([], lane_c, tile_c, [12]), (0.9, 2.5, 4.0, 5.9)):
+            ops.add(kind, q, ctrl, angle=ang + 0.01 * q)
+    arr = ops.array()
+    (ref, _), (comp, passes) = run_modes(n, st, arr, (RUN_PER_GATE, RUN_COMPILED))
+    assert passes == 1
+    d = float(np.max(np.abs(comp - ref)))
+    assert d <= 1e-13, (kind, d)
+
+
+def test_compiled_diagonal_masks_in_one_pass():
+    """Z, P and RZ with targets outside the tile, on register, lane and wave
+    bits, under control sets mixing lane, register, wave and outside bits."""
+    print("compiled diagonal masks", flush=True)
+    n = 13
+    rng = np.random.default_rng(77)
+    st = random_state(rng, n)
+    cases = [(11, []), (11, [0]), (11, [3]), (11, [7]), (11, [9]), (11, [12]), (11, [0, 3, 12]),
+             (3, []), (3, [11]), (3, [0, 12]), (4, [1, 5]), (5, [8, 10]),
+             (0, []), (0, [3, 4]), (1, [11, 12]), (7, [2, 9]), (9, []), (10, [3, 6, 11]), (12, [4, 10])]
+    ops = circuits.OpList()
+    for t in range(3, 11):
+        ops.add(0, t)
+    for kind in (2, 4, 7):
+        for t, ctrl in cases:
+            ops.add(kind, t, ctrl, angle=0.3 + 0.1 * t)
+    arr = ops.array()
+    (ref, _), (comp, passes) = run_modes(n, st, arr, (RUN_PER_GATE, RUN_COMPILED))
+    assert passes == 1
+    assert float(np.max(np.abs(comp - ref))) <= 1e-12
+
+
+@pytest.mark.parametrize("n,layers,seed", [(16, 6, 3), (20, 4, 4)])
+def test_compiled_brickwork_against_oracle_and_reorder(n, layers, seed):
+    """The brickwork circuit (every gate kind, CZ/CNOT, multi-controlled X)
+    from |0>: amplitudes against the oracle's program-order run to 1e-12, and
+    against KBWH_RUN_REORDER's generic passes to 1e-13, with the same passes."""
+    print(f"compiled brickwork n={n}", flush=True)
+    ops = circuits.OpList()
+    circuits.brickwork_layers(ops, np.random.default_rng(seed), n, layers)
+    arr = ops.array()
+    o = oracle.DenseV2(n)
+    o.run(arr)
+    want = o.state()
+    g = DenseHip(n)
+    g.run(arr, mode=RUN_COMPILED)
+    comp, passes = g.state(), g.run_stats()[1]
+    g.clear()
+    g.run(arr, mode=RUN_REORDER)
+    reo, passes_r = g.state(), g.run_stats()[1]
+    g.close()
+    assert passes == passes_r
+    assert float(np.max(np.abs(comp - want))) <= 1e-12
+    assert float(np.max(np.abs(comp - reo))) <= 1e-13
+
+
+def test_compiled_kernels_are_reused_for_new_angles():
+    """A second run of the same circuit structure with other parameters
+    compiles nothing and finds every pass's kernel; the energies of both
+    parameter sets match the oracle."""
+    print("compiled reuse", flush=True)
+    n = 14
+    ops, params = circuits.hea(n, 4, seed=21)
+    ham = circuits.xxz(n, seed=22)
+    g = DenseHip(n)
+    g.run(ops, params, mode=RUN_COMPILED)
+    e1 = g.exp_value(ham)
+    passes = g.run_stats()[1]
+    before = compiled_stats()
+    p2 = np.random.default_rng(23).uniform(-math.pi, math.pi, size=len(params))
+    g.clear()
+    g.run(ops, p2, mode=RUN_COMPILED)
+    e2 = g.exp_value(ham)
+    after = compiled_stats()
+    assert after["compiled"] == before["compiled"]
+    assert after["cache_hits"] - before["cache_hits"] == passes
+    for pp, e in ((params, e1), (p2, e2)):
+        o = oracle.DenseV2(n)
+        o.run(ops, pp)
+        want = o.exp_value(ham.pauli_strings, ham.coefficients)
+        assert abs(e - want) <= 1e-10 * max(1.0, abs(want)), (e, want)
+    g.close()
+
+
+def test_compiled_long_pass_runs_the_generic_kernel():
+    """A pass of more than 256 gates (X gates on qubit 0 under changing
+    controls all fit one tile) runs the generic kernel: same result as
+    KBWH_RUN_REORDER, one more pass counted as generic."""
+    print("compiled long pass", flush=True)
+    n = 12
+    rng = np.random.default_rng(5)
+    st = random_state(rng, n)
+    ops = circuits.OpList()
+    for i in range(300):
+        ops.add(6, i % 3, [3 + (i % 9)], angle=0.01 * i)
+    arr = ops.array()
+    before = compiled_stats()
+    (comp, passes), (reo, _) = run_modes(n, st, arr, (RUN_COMPILED, RUN_REORDER))
+    after = compiled_stats()
+    assert passes == 1
+    assert after["generic_passes"] - before["generic_passes"] == 1
+    assert np.array_equal(comp, reo)

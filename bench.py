@@ -7,11 +7,18 @@ strings.  A step is one pass of the circuit over the resident state; the
 state stays in HBM across steps (gates are unitary, so the circuit can be
 re-applied).  value = gates/s over the timed steps.
 
-roofline: the dominant kernel is the non-diagonal pair pass (X/Y/H/RX/RY with
-no control): per launch it reads and writes every amplitude once,
-2^29 pairs x 64 B = 34.36 GB; `achieved` divides that by the average
-duration of those launches, timed with HIP events on the engine's own
-stream inside the timed region.
+Value path: the fastest of the program-order tiled passes (bit-exact), the
+dependency-ordered passes on the generic kernel and the same passes on
+kernels compiled for each pass (both within 1e-12 of the program-order
+state, checked over all 2^30 amplitudes before timing); config.run_mode
+names it.  The compiled kernels are built by the first run (compile time
+reported in "compiled.compile", outside the timed steps).
+
+roofline: the value path's pass kernel: per launch it reads and writes every
+amplitude once, 2^30 x 32 B = 34.36 GB; `achieved` divides that by the
+average launch duration (the timed steps' HIP events on the engine's own
+stream, divided by the passes per step).  The per-gate path's pair kernel
+has its own roofline under "per_gate".
 
 cpu_baseline: the oracle (the C restatement of the reference's DenseV2 with
 OpenMP in place of rayon) on the same 30-qubit state and the same gates,
@@ -43,7 +50,7 @@ HBM_PEAK_GBPS = 8000.0
 TRAFFIC_PAIR = 34362315175.0
 TRAFFIC_TILE = 34369246667.0
 TRAFFIC_TILE_FMA = 34359974941.0
-TRAFFIC_COMPILED = None  # not yet counted
+TRAFFIC_COMPILED = 34363556768.0
 
 
 def parse():

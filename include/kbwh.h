@@ -292,8 +292,37 @@ int32_t kbwh_batch_run(kbwh_batch *b, const double *params, double *exp_values);
 int32_t kbwh_batch_use_graph(kbwh_batch *b, int32_t enable);
 /* The first `count` amplitudes of circuit `circuit` after the last run. */
 int32_t kbwh_batch_state(kbwh_batch *b, uint32_t circuit, uint64_t count, double *out);
-/* Gate records, tiled passes and amplitude-passes (sum over circuits of 2^n x passes). */
+/* Gate records, tiled passes and amplitude-passes (sum over circuits of 2^n x passes; a group's
+ * pass or read sweep counts once for all its sets, with 2^n x sets amplitudes). */
 int32_t kbwh_batch_stats(const kbwh_batch *b, uint64_t *gates, uint64_t *passes, uint64_t *amplitudes);
+/* Structure groups (no counterpart in the reference).  Circuits of a batch
+ * whose ops (kinds, targets, controls, parameter slots, fixed angles) and
+ * Pauli strings are the same, differing only in parameters and string
+ * weights, run as one state of sets when the group spans at least 256 tiles
+ * of 2^11 amplitudes: one launch per dependency-ordered pass over every set,
+ * then read sweeps with per-set weights.  The other circuits run in the
+ * persistent kernel.  A group is planned with at most 8 non-diagonal targets
+ * above qubit 2 per pass and runs the generic pass kernel until configured. */
+typedef struct kbwh_batch_group_info {
+    uint32_t num_qubits;    /* state width of each set: max(n, 11) */
+    uint32_t num_sets;      /* circuits in the group */
+    uint32_t tile_targets;  /* target cap of its passes */
+    uint32_t compiled;      /* passes run by pass-specialised kernels */
+    uint32_t passes, sweeps;
+    double tuned_ms;        /* the chosen configuration's time in the last kbwh_batch_autotune (0: not tuned) */
+} kbwh_batch_group_info;
+int32_t kbwh_batch_groups(const kbwh_batch *b, kbwh_batch_group_info *info, uint32_t capacity,
+                          uint32_t *num_groups);
+/* Every group: target cap 1..8 and, with compiled != 0, pass-specialised
+ * kernels compiled at run time (as KBWH_RUN_COMPILED; a pass that does not
+ * compile keeps the generic kernel).  Drops a captured graph. */
+int32_t kbwh_batch_configure(kbwh_batch *b, uint32_t tile_targets, int32_t compiled);
+/* Batch autotuner: per group, times its run with the generic kernel at
+ * target caps 8, 6 and 4, then with compiled kernels at the best cap, and
+ * keeps the fastest.  params: the parameter vector of the timing runs.
+ * ms[4 g + i] receives candidate i's time for group g (capacity entries).
+ * The batch's states are overwritten; drops a captured graph. */
+int32_t kbwh_batch_autotune(kbwh_batch *b, const double *params, double *ms, uint32_t capacity);
 int32_t kbwh_batch_free(kbwh_batch *b);
 
 /* ---- raw access (tests, tooling) ------------------------------------- */

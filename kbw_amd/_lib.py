@@ -36,6 +36,9 @@ CIRCUIT_DTYPE = np.dtype([("num_qubits", "<u4"), ("num_strings", "<u4"), ("first
 assert CIRCUIT_DTYPE.itemsize == 32
 
 # kbwh_tune_result, 80 bytes
+GROUP_DTYPE = np.dtype([("num_qubits", "<u4"), ("num_sets", "<u4"), ("tile_targets", "<u4"), ("compiled", "<u4"),
+                        ("passes", "<u4"), ("sweeps", "<u4"), ("tuned_ms", "<f8")])
+assert GROUP_DTYPE.itemsize == 32
 TUNE_DTYPE = np.dtype([("mode", "<u4"), ("fusion_width", "<u4"), ("tile_targets", "<u4"),
                        ("num_candidates", "<u4"), ("ms", "<f8", (8,))], align=True)
 assert TUNE_DTYPE.itemsize == 80
@@ -80,6 +83,9 @@ SIGNATURES = {
     "kbwh_batch_use_graph": [_vp, _i32],
     "kbwh_batch_state": [_vp, _u32, _u64, _p],
     "kbwh_batch_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)],
+    "kbwh_batch_groups": [_vp, _p, _u32, C.POINTER(_u32)],
+    "kbwh_batch_configure": [_vp, _u32, _i32],
+    "kbwh_batch_autotune": [_vp, _p, _p, _u32],
     "kbwh_batch_free": [_vp],
     "kbwh_build_info": [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t)],
     "kbwh_last_error": [],

@@ -76,6 +76,28 @@ class Batch:
         check(self._lib.kbwh_batch_stats(self._h, C.byref(g), C.byref(p), C.byref(a)))
         return g.value, p.value, a.value
 
+    def groups(self):
+        """The structure groups run as sets: a list of dicts (num_qubits,
+        num_sets, tile_targets, compiled passes, passes, sweeps, tuned_ms)."""
+        n = C.c_uint32()
+        check(self._lib.kbwh_batch_groups(self._h, None, 0, C.byref(n)))
+        info = np.zeros(n.value, dtype=_lib.GROUP_DTYPE)
+        check(self._lib.kbwh_batch_groups(self._h, ptr(info), n.value, C.byref(n)))
+        return [{k: info[k][i].item() for k in info.dtype.names} for i in range(len(info))]
+
+    def configure(self, tile_targets=8, compiled=False):
+        check(self._lib.kbwh_batch_configure(self._h, tile_targets, 1 if compiled else 0))
+
+    def autotune(self, params):
+        """Times every group's candidates (generic kernel at 8, 6 and 4
+        targets per pass, compiled kernels at the best) and keeps the fastest;
+        returns the candidates' times, one row of 4 per group (ms)."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        g = len(self.groups())
+        ms = np.zeros(max(4 * g, 1))
+        check(self._lib.kbwh_batch_autotune(self._h, ptr(p), ptr(ms), 4 * g))
+        return ms[:4 * g].reshape(g, 4)
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.kbwh_batch_free(self._h)

@@ -30,6 +30,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <string>
 #include <numeric>
 #include <vector>
 
@@ -78,6 +80,9 @@ constexpr uint32_t kMaxChunks = 16;
 // workgroup then goes on, the error flag is set and the host reports it, so
 // the kernel ends even if a chunk never arrived.
 constexpr uint32_t kSpinLimit = 1u << 24;
+// A structure group runs as sets when its launches cover at least this many
+// tiles (one per CU): smaller groups stay in the persistent kernel.
+constexpr uint64_t kMinGroupTiles = 256;
 
 struct BPass {
     TileSpec ts;
@@ -280,6 +285,159 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WA
     }
 }
 
+// ---- groups of circuits that share one structure ----------------------
+//
+// Circuits whose gates, controls, parameter slots and Pauli strings are the
+// same (only parameters and string weights differ) run as one state of
+// sets: S sets of 2^n_eff amplitudes one after another, one launch per pass
+// over all of them (per-set gate records: tile t takes the records of set
+// t >> (n_eff - TL)), then read sweeps for the strings with per-set weights.
+// Every launch covers S x 2^(n_eff - TL) tiles, so large groups fill the
+// device the way one large state does, with no barriers between workgroups.
+
+// per-set records: out[dst[r] + s stride[r]] = template r with the
+// coefficients of set s's parameters (pad: parameter slot relative to the
+// circuit's first parameter, or -1)
+__global__ __launch_bounds__(256) void k_expand_group(const TGate *__restrict__ tmpl, const uint32_t *__restrict__ dst,
+                                                      const uint32_t *__restrict__ stride, uint32_t nrec,
+                                                      uint32_t nsets, const uint32_t *__restrict__ pbase,
+                                                      const double *__restrict__ params, TGate *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint64_t)nrec * nsets) return;
+    const uint32_t s = (uint32_t)(i / nrec), r = (uint32_t)(i % nrec);
+    TGate g = tmpl[r];
+    const int32_t p = (int32_t)g.pad;
+    if (p >= 0) set_coefficients_dev(g, params[(uint64_t)pbase[s] + (uint32_t)p]);
+    out[dst[r] + (uint64_t)s * stride[r]] = g;
+}
+
+// one pass of a group with the generic gate loop (passes without a compiled
+// kernel); INIT: the tile starts as |0...0> of its set instead of a load
+template <bool INIT>
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_gpass(
+    dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gates, uint32_t ng, uint32_t ntiles,
+    uint32_t set_shift, uint64_t set_mask) {
+    __shared__ dv2 lds[TT * kChunk];
+    const uint32_t tid = threadIdx.x, tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    gates += (size_t)(tile >> set_shift) * ng;
+    const uint64_t base = tile_base(tile, ts);
+    const uint64_t dt = thread_offset(base, tid, ts);
+    dv2 v[TR];
+    if constexpr (INIT)
+        tile_zero_state(v, dt, ts, set_mask);
+    else
+        tile_load(v, a, dt, ts);
+    if (ng) {
+        const TGate *gp = gates, *const ge = gates + ng;
+        do {
+            const TGate G = *gp;
+            apply_gate(v, lds, base, tid, G);
+        } while (++gp != ge);
+    }
+    tile_store(v, a, dt, ts);
+}
+
+// one read sweep: the strings [first, first + ns) of the group's table, whose
+// X/Y qubits are tile qubits of ts, weighted per set (coef[set * nstr + k]);
+// partial[tile] = the tile's sum (arithmetic as in k_batch)
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, KBWH_TILE_WAVES))) void k_gsweep(
+    const dv2 *__restrict__ a, TileSpec ts, const BString *__restrict__ strings, uint32_t first, uint32_t ns,
+    uint32_t has_x, uint32_t ntiles, uint32_t set_shift, const double *__restrict__ coef, uint32_t nstr,
+    double *__restrict__ partial) {
+    __shared__ dv2 lds[TT * TR];
+    __shared__ double ws[TT / 64];
+    const uint32_t tid = threadIdx.x, tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const double *cw = coef + (size_t)(tile >> set_shift) * nstr + first;
+    const uint64_t dt = thread_offset(tile_base(tile, ts), tid, ts);
+    dv2 v[TR];
+    tile_load(v, a, dt, ts);
+    if (has_x) {
+#pragma unroll
+        for (int r = 0; r < TR; ++r) lds[tid | (r << TB)] = v[r];
+        __syncthreads();
+    }
+    double acc = 0.0;
+    for (uint32_t k = 0; k < ns; ++k) {
+        const BString S = strings[first + k];
+        uint32_t zr = 0;
+#pragma unroll
+        for (int b = 0; b < TL - TB; ++b)
+            if (ts.qbit[TB + b] & S.z) zr |= 1u << b;
+        const bool pd = (__popcll(dt & S.z) & 1) != 0;
+        double sk = 0.0;
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+            bool flip = pd != ((__builtin_popcount((uint32_t)r & zr) & 1) != 0);
+            double term;
+            if (S.xt == 0) {
+                term = v[r].x * v[r].x + v[r].y * v[r].y;
+            } else {
+                const dv2 o = lds[(tid | (r << TB)) ^ S.xt], m = v[r];
+                const double wr = o.x * m.x + o.y * m.y;
+                const double wi = o.x * m.y - o.y * m.x;
+                const uint32_t q = S.ny & 3u;
+                term = (q & 1u) ? wi : wr;
+                flip = flip != (((q + 1u) & 2u) != 0);
+            }
+            sk += flip ? -term : term;
+        }
+        acc += cw[k] * sk;
+    }
+    const double total = block_sum(acc, ws);
+    if (tid == 0) partial[tile] = total;
+}
+
+// one workgroup per set: the set's tiles over every sweep, in a fixed order
+// (each thread a fixed stride of tiles, then the block sum), into the
+// circuit's result
+__global__ __launch_bounds__(256) void k_gsum(const double *__restrict__ partial, uint32_t nsweeps, uint32_t ntiles,
+                                              uint32_t tiles_per_set, const uint32_t *__restrict__ circ,
+                                              double *__restrict__ results) {
+    __shared__ double ws[256 / 64];
+    const uint32_t s = blockIdx.x;
+    double acc = 0.0;
+    for (uint32_t g = 0; g < nsweeps; ++g) {
+        const double *p = partial + (size_t)g * ntiles + (size_t)s * tiles_per_set;
+        for (uint32_t t = threadIdx.x; t < tiles_per_set; t += 256) acc += p[t];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) results[circ[s]] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+struct GPass {
+    TileSpec ts;
+    uint32_t first, count;  // template records [first, first + count)
+    uint32_t flags;         // kJitInit on the first pass
+    uint32_t pad;
+};
+
+struct GSweep {
+    TileSpec ts;
+    uint32_t first, count, has_x, pad;
+};
+
+struct SetGroup {
+    uint32_t n_eff = 0, nsets = 0, cap = TL - 3;
+    bool compiled = false;
+    double tuned_ms = 0.0;  // the chosen configuration's time in the last tuning (0: not tuned)
+    uint64_t state_off = 0;
+    std::vector<uint32_t> circ;  // circuit of each set
+    std::vector<GPass> passes;
+    std::vector<TGate> tmpl;
+    std::vector<GSweep> sweeps;
+    uint32_t nstr = 0;
+    TGate *d_tmpl = nullptr, *d_recs = nullptr;
+    uint32_t *d_meta = nullptr;  // dst[nrec], stride[nrec], pbase[nsets], circ[nsets]
+    BString *d_str = nullptr;
+    double *d_coef = nullptr, *d_partial = nullptr;
+    std::vector<void *> fns;  // compiled kernel of each pass (null: k_gpass)
+};
+
 }  // namespace
 
 struct kbwh_batch {
@@ -304,16 +462,26 @@ struct kbwh_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t grid = 0;
+    std::vector<SetGroup> groups;
+    int device = 0;
+    // the inputs, kept for re-planning groups (kbwh_batch_configure / _autotune)
+    std::vector<kbwh_circuit> in_circuits;
+    std::vector<kbwh_op> in_ops;
+    std::vector<uint8_t> in_paulis;
+    std::vector<uint32_t> in_qubits, in_offsets;
+    std::vector<double> in_coeffs;
 };
 
 namespace {
+
+hipError_t enqueue_group(const SetGroup &G, dv2 *states, const double *params, double *results, hipStream_t st);
 
 int32_t enqueue_run(kbwh_batch *b) {
     hipStream_t st = b->stream;
     hipError_t e = hipSuccess;
     if (b->nparams)
         e = hipMemcpyAsync(b->d_params, b->h_params, b->nparams * 8, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && b->ngates) {
+    if (e == hipSuccess && b->ngates && b->nitems) {
         k_coef<<<(uint32_t)((b->ngates + 255) / 256), 256, 0, st>>>(b->d_tmpl, b->d_live, b->ngates, b->d_params);
         e = hipGetLastError();
     }
@@ -322,11 +490,13 @@ int32_t enqueue_run(kbwh_batch *b) {
         k_reset<<<std::min<uint32_t>((nctl + 255) / 256, 64), 256, 0, st>>>(b->d_ctl, nctl);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) {
+    if (e == hipSuccess && b->nitems) {
         k_batch<<<b->grid, TT, 0, st>>>(b->states, b->d_circs, b->d_items, b->nitems, b->d_passes, b->d_live,
                                        b->d_strings, b->d_results, b->d_partial, b->d_ctl, b->ncirc);
         e = hipGetLastError();
     }
+    for (size_t g = 0; g < b->groups.size() && e == hipSuccess; ++g)
+        e = enqueue_group(b->groups[g], b->states, b->d_params, b->d_results, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->h_results, b->d_results, b->ncirc * 8, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess)
@@ -334,9 +504,12 @@ int32_t enqueue_run(kbwh_batch *b) {
     return hip_status(e, "batch run");
 }
 
+void free_group(SetGroup &G);
+
 void destroy(kbwh_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (SetGroup &G : b->groups) free_group(G);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     void *dev[] = {b->states, b->d_circs, b->d_items, b->d_partial, b->d_passes, b->d_tmpl, b->d_live,
@@ -546,6 +719,246 @@ int32_t build_plan(const kbwh_circuit *circuits, uint32_t num_circuits, const kb
     return KBWH_OK;
 }
 
+// The structure of a circuit: width, ops (kind, target, controls, parameter
+// slot, and the angle of an unparameterised op) and the letters and qubits of
+// its Pauli strings.  Circuits with equal keys differ only in parameters and
+// string weights.
+std::string circuit_key(const kbwh_circuit &K, const kbwh_op *ops, const uint8_t *paulis, const uint32_t *qubits,
+                        const uint32_t *offsets) {
+    std::string k;
+    auto put = [&](const void *p, size_t n) { k.append((const char *)p, n); };
+    put(&K.num_qubits, sizeof K.num_qubits);
+    put(&K.num_ops, sizeof K.num_ops);
+    put(&K.num_strings, sizeof K.num_strings);
+    for (uint64_t i = K.first_op; i < K.first_op + K.num_ops; ++i) {
+        const kbwh_op &op = ops[i];
+        put(&op.kind, sizeof op.kind);
+        put(&op.target, sizeof op.target);
+        put(&op.ctrl_mask, sizeof op.ctrl_mask);
+        put(&op.param, sizeof op.param);
+        // a fixed angle counts for P, RX, RY and RZ only (X, Y, Z and H
+        // ignore it), by value (-0.0 and 0.0 are one key)
+        if (op.param < 0 && (op.kind == KP || op.kind == KRX || op.kind == KRY || op.kind == KRZ)) {
+            const double a = op.angle == 0.0 ? 0.0 : op.angle;
+            put(&a, sizeof a);
+        }
+    }
+    for (uint32_t j = K.first_string; j < K.first_string + K.num_strings; ++j) {
+        const uint32_t len = offsets[j + 1] - offsets[j];
+        put(&len, sizeof len);
+        for (uint32_t t = offsets[j]; t < offsets[j + 1]; ++t) {
+            put(&paulis[t], 1);
+            put(&qubits[t], sizeof qubits[t]);
+        }
+    }
+    return k;
+}
+
+// Plans a group's passes (dependency order, at most `cap` non-diagonal
+// targets above qubit 2 per pass) and read sweeps from its first circuit.
+// Inputs were checked by build_plan.
+int32_t plan_group(SetGroup &G, uint32_t cap, const kbwh_circuit *circuits, const kbwh_op *ops, const uint8_t *paulis,
+                   const uint32_t *qubits, const uint32_t *offsets, const double *coeffs) {
+    const kbwh_circuit &K = circuits[G.circ[0]];
+    const uint32_t n = G.n_eff;
+    G.cap = cap;
+    G.passes.clear();
+    G.tmpl.clear();
+    G.sweeps.clear();
+    std::vector<PhysOp> phys;
+    std::vector<int32_t> slot;
+    for (uint64_t i = K.first_op; i < K.first_op + K.num_ops; ++i) {
+        const kbwh_op &op = ops[i];
+        phys.push_back(PhysOp{(int)op.kind, op.target, op.ctrl_mask & ~(1ull << op.target), op.angle});
+        slot.push_back(op.param);
+    }
+    std::vector<uint32_t> order, ends;
+    schedule_passes(phys, n, cap, order, ends);
+    if (order.size() != phys.size()) return set_error(KBWH_ERR_DEVICE, "group schedule incomplete");
+    size_t k0 = 0;
+    for (const uint32_t end : ends) {
+        Pass cur;
+        for (size_t k = k0; k < end; ++k) add_op(cur, phys[order[k]]);
+        int pos_of[64];
+        uint32_t q_at[TL];
+        assign_positions(n, cur, pos_of, q_at);
+        GPass P;
+        memset(&P, 0, sizeof P);
+        P.ts = make_tile_spec(q_at);
+        P.first = (uint32_t)G.tmpl.size();
+        P.count = (uint32_t)(end - k0);
+        for (size_t k = k0; k < end; ++k) {
+            TGate g = make_tgate(phys[order[k]], pos_of, n);
+            g.pad = (uint32_t)slot[order[k]];
+            G.tmpl.push_back(g);
+        }
+        G.passes.push_back(P);
+        k0 = end;
+    }
+    if (G.passes.empty()) {
+        // no gates: one pass makes and stores |0...0>
+        uint32_t q_at[TL];
+        for (int b = 0; b < TL; ++b) q_at[b] = (uint32_t)b;
+        GPass P;
+        memset(&P, 0, sizeof P);
+        P.ts = make_tile_spec(q_at);
+        G.passes.push_back(P);
+    }
+    G.passes[0].flags = kJitInit;
+    // read sweeps: qubits 0..2, then the X/Y qubits of the remaining strings
+    // in order while they fit, then the lowest free qubits
+    std::vector<uint64_t> xs(K.num_strings);
+    for (uint32_t j = 0; j < K.num_strings; ++j) {
+        uint64_t x = 0;
+        for (uint32_t t = offsets[K.first_string + j]; t < offsets[K.first_string + j + 1]; ++t)
+            if (paulis[t] == KBWH_PAULI_X || paulis[t] == KBWH_PAULI_Y) x |= 1ull << qubits[t];
+        xs[j] = x;
+    }
+    std::vector<bool> done(K.num_strings, false);
+    std::vector<uint32_t> sorder;  // string of each table entry, in sweep order
+    for (;;) {
+        uint64_t qset = 7;
+        bool any = false;
+        for (uint32_t j = 0; j < K.num_strings; ++j) {
+            if (done[j]) continue;
+            any = true;
+            if (__builtin_popcountll(qset | xs[j]) <= TL) qset |= xs[j];
+        }
+        if (!any) break;
+        for (uint32_t q = 3; q < n && __builtin_popcountll(qset) < TL; ++q) qset |= 1ull << q;
+        uint32_t q_at[TL];
+        int b = 0;
+        for (uint32_t q = 0; q < n; ++q)
+            if (qset >> q & 1ull) q_at[b++] = q;
+        GSweep W;
+        memset(&W, 0, sizeof W);
+        W.ts = make_tile_spec(q_at);
+        W.first = (uint32_t)sorder.size();
+        for (uint32_t j = 0; j < K.num_strings; ++j) {
+            if (done[j] || (xs[j] & ~qset)) continue;
+            sorder.push_back(j);
+            done[j] = true;
+            if (xs[j]) W.has_x = 1;
+        }
+        W.count = (uint32_t)sorder.size() - W.first;
+        if (W.count == 0) return set_error(KBWH_ERR_INVALID_ARGUMENT, "plan: string placement");
+        G.sweeps.push_back(W);
+    }
+    G.nstr = (uint32_t)sorder.size();
+    // device tables: records, metadata, strings, per-set weights
+    const uint32_t S = G.nsets, nrec = (uint32_t)G.tmpl.size();
+    std::vector<uint32_t> meta(2 * (size_t)nrec + 2 * (size_t)S);
+    for (const GPass &P : G.passes)
+        for (uint32_t r = P.first; r < P.first + P.count; ++r) {
+            meta[r] = S * P.first + (r - P.first);
+            meta[nrec + r] = P.count;
+        }
+    std::vector<BString> strs;
+    for (uint32_t e = 0; e < G.nstr; ++e) {
+        const uint32_t j = sorder[e];
+        BString B{0, 0, 0, 0, 0.0};
+        for (uint32_t t = offsets[K.first_string + j]; t < offsets[K.first_string + j + 1]; ++t) {
+            const uint64_t bit = 1ull << qubits[t];
+            if (paulis[t] == KBWH_PAULI_X) B.x |= bit;
+            else if (paulis[t] == KBWH_PAULI_Y) { B.x |= bit; B.z |= bit; B.ny++; }
+            else B.z |= bit;
+        }
+        strs.push_back(B);
+    }
+    for (const GSweep &W : G.sweeps)
+        for (uint32_t e = W.first; e < W.first + W.count; ++e)
+            for (int b = 0; b < TL; ++b)
+                if (strs[e].x & W.ts.qbit[b]) strs[e].xt |= 1u << b;
+    std::vector<double> coef((size_t)S * std::max<uint32_t>(G.nstr, 1), 0.0);
+    for (uint32_t s = 0; s < S; ++s) {
+        const kbwh_circuit &C = circuits[G.circ[s]];
+        meta[2 * (size_t)nrec + s] = C.first_param;
+        meta[2 * (size_t)nrec + S + s] = G.circ[s];
+        for (uint32_t e = 0; e < G.nstr; ++e) coef[(size_t)s * G.nstr + e] = coeffs[C.first_string + sorder[e]];
+    }
+    const uint64_t ntiles = (uint64_t)S << (n - TL);
+    void *bufs[] = {G.d_tmpl, G.d_recs, G.d_meta, G.d_str, G.d_coef, G.d_partial};
+    for (void *q : bufs)
+        if (q) (void)hipFree(q);
+    G.d_tmpl = G.d_recs = nullptr;
+    G.d_meta = nullptr;
+    G.d_str = nullptr;
+    G.d_coef = G.d_partial = nullptr;
+    hipError_t e = hipMalloc(&G.d_tmpl, std::max<size_t>(nrec, 1) * sizeof(TGate));
+    if (e == hipSuccess && nrec) e = hipMemcpy(G.d_tmpl, G.tmpl.data(), nrec * sizeof(TGate), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&G.d_recs, std::max<size_t>((size_t)nrec * S, 1) * sizeof(TGate));
+    if (e == hipSuccess) e = hipMalloc(&G.d_meta, meta.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(G.d_meta, meta.data(), meta.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&G.d_str, std::max<size_t>(strs.size(), 1) * sizeof(BString));
+    if (e == hipSuccess && !strs.empty())
+        e = hipMemcpy(G.d_str, strs.data(), strs.size() * sizeof(BString), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&G.d_coef, coef.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(G.d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&G.d_partial, std::max<uint64_t>(ntiles * G.sweeps.size(), 1) * sizeof(double));
+    G.fns.assign(G.passes.size(), nullptr);
+    return hip_status(e, "batch group tables");
+}
+
+// Compiles (or drops) the group's pass-specialised kernels.
+int32_t group_kernels(SetGroup &G, int device, bool compiled) {
+    G.compiled = compiled;
+    G.fns.assign(G.passes.size(), nullptr);
+    if (!compiled) return KBWH_OK;
+    std::vector<JitPass> lst;
+    for (const GPass &P : G.passes) lst.push_back(JitPass{G.tmpl.data() + P.first, P.count, P.flags});
+    return jit_prepare(device, lst, G.fns);
+}
+
+void free_group(SetGroup &G) {
+    void *bufs[] = {G.d_tmpl, G.d_recs, G.d_meta, G.d_str, G.d_coef, G.d_partial};
+    for (void *q : bufs)
+        if (q) (void)hipFree(q);
+    G.d_tmpl = G.d_recs = nullptr;
+    G.d_meta = nullptr;
+    G.d_str = nullptr;
+    G.d_coef = G.d_partial = nullptr;
+}
+
+// A group's run: per-set records, its passes, its sweeps, per-set energies.
+hipError_t enqueue_group(const SetGroup &G, dv2 *states, const double *params, double *results, hipStream_t st) {
+    const uint32_t S = G.nsets, nrec = (uint32_t)G.tmpl.size(), n = G.n_eff;
+    const uint32_t shift = n - TL, ntiles = S << shift;
+    const uint64_t set_mask = (1ull << n) - 1;
+    dv2 *a = states + G.state_off;
+    hipError_t e = hipSuccess;
+    if (nrec) {
+        const uint64_t total = (uint64_t)nrec * S;
+        k_expand_group<<<(uint32_t)((total + 255) / 256), 256, 0, st>>>(G.d_tmpl, G.d_meta, G.d_meta + nrec, nrec, S,
+                                                                       G.d_meta + 2 * (size_t)nrec, params, G.d_recs);
+        e = hipGetLastError();
+    }
+    for (size_t p = 0; p < G.passes.size() && e == hipSuccess; ++p) {
+        const GPass &P = G.passes[p];
+        const TGate *recs = G.d_recs + (size_t)S * P.first;
+        if (G.fns[p]) {
+            e = jit_launch(G.fns[p], (double2 *)a, &P.ts, recs, ntiles, st, shift, set_mask);
+        } else {
+            if (P.flags & kJitInit)
+                k_gpass<true><<<ntiles, TT, 0, st>>>(a, P.ts, recs, P.count, ntiles, shift, set_mask);
+            else
+                k_gpass<false><<<ntiles, TT, 0, st>>>(a, P.ts, recs, P.count, ntiles, shift, set_mask);
+            e = hipGetLastError();
+        }
+    }
+    for (size_t g = 0; g < G.sweeps.size() && e == hipSuccess; ++g) {
+        const GSweep &W = G.sweeps[g];
+        k_gsweep<<<ntiles, TT, 0, st>>>(a, W.ts, G.d_str, W.first, W.count, W.has_x, ntiles, shift, G.d_coef, G.nstr,
+                                        G.d_partial + (size_t)g * ntiles);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        k_gsum<<<S, 256, 0, st>>>(G.d_partial, (uint32_t)G.sweeps.size(), ntiles, 1u << shift,
+                                  G.d_meta + 2 * (size_t)nrec + S, results);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
 }  // namespace
 
 extern "C" {
@@ -581,9 +994,49 @@ int32_t kbwh_batch_new(const kbwh_circuit *circuits, uint32_t num_circuits, cons
     std::vector<TGate> &tmpl = pl.tmpl;
     std::vector<BString> &strings = pl.strings;
     const uint64_t state_total = pl.state_total;
+    // circuits of one structure form groups of sets (large enough to fill
+    // the device); the others run in the persistent kernel
+    std::map<std::string, std::vector<uint32_t>> by_key;
+    for (uint32_t c = 0; c < num_circuits; ++c)
+        by_key[circuit_key(circuits[c], ops, paulis, qubits, offsets)].push_back(c);
+    std::vector<SetGroup> groups;
+    std::vector<uint8_t> grouped(num_circuits, 0);
+    for (const auto &kv : by_key) {
+        const std::vector<uint32_t> &list = kv.second;
+        const uint32_t shift = circs[list[0]].n_eff - TL;
+        // a launch covers at most 2^24 - 1 tiles of 256 threads
+        const size_t max_sets = std::max<size_t>(1, ((1u << 24) - 1) >> shift);
+        for (size_t i0 = 0; i0 < list.size(); i0 += max_sets) {
+            const size_t cnt = std::min(max_sets, list.size() - i0);
+            if (cnt < 2 || ((uint64_t)cnt << shift) < kMinGroupTiles) continue;
+            SetGroup G;
+            G.n_eff = circs[list[0]].n_eff;
+            G.nsets = (uint32_t)cnt;
+            G.circ.assign(list.begin() + i0, list.begin() + i0 + cnt);
+            for (const uint32_t c : G.circ) grouped[c] = 1;
+            groups.push_back(std::move(G));
+        }
+    }
+    // state layout: the persistent kernel's circuits, then each group's sets
+    {
+        uint64_t off = 0;
+        for (uint32_t c = 0; c < num_circuits; ++c)
+            if (!grouped[c]) {
+                circs[c].state_off = off;
+                off += 1ull << circs[c].n_eff;
+            }
+        for (SetGroup &G : groups) {
+            G.state_off = off;
+            for (const uint32_t c : G.circ) {
+                circs[c].state_off = off;
+                off += 1ull << G.n_eff;
+            }
+        }
+    }
     // largest circuits first: the persistent workgroups balance the tail
-    std::vector<uint32_t> order(num_circuits);
-    std::iota(order.begin(), order.end(), 0u);
+    std::vector<uint32_t> order;
+    for (uint32_t c = 0; c < num_circuits; ++c)
+        if (!grouped[c]) order.push_back(c);
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t x, uint32_t y) { return circs[x].n_eff > circs[y].n_eff; });
     kbwh_batch *b = new kbwh_batch();
@@ -634,6 +1087,32 @@ int32_t kbwh_batch_new(const kbwh_circuit *circuits, uint32_t num_circuits, cons
         int32_t rc = hip_status(e, "kbwh_batch_new");
         destroy(b);
         return rc;
+    }
+    b->device = dev;
+    b->groups = std::move(groups);
+    if (!b->groups.empty()) {
+        uint64_t nops = 0;
+        uint32_t nstr = 0;
+        for (uint32_t c = 0; c < num_circuits; ++c) {
+            nops = std::max<uint64_t>(nops, circuits[c].first_op + circuits[c].num_ops);
+            nstr = std::max<uint32_t>(nstr, circuits[c].first_string + circuits[c].num_strings);
+        }
+        const uint32_t nlet = nstr ? offsets[nstr] : 0;
+        b->in_circuits.assign(circuits, circuits + num_circuits);
+        b->in_ops.assign(ops, ops + nops);
+        if (nlet) {
+            b->in_paulis.assign(paulis, paulis + nlet);
+            b->in_qubits.assign(qubits, qubits + nlet);
+        }
+        b->in_offsets.assign(offsets, offsets + nstr + 1);
+        if (nstr) b->in_coeffs.assign(coeffs, coeffs + nstr);
+    }
+    for (SetGroup &G : b->groups) {
+        int32_t rc = plan_group(G, TL - 3, circuits, ops, paulis, qubits, offsets, coeffs);
+        if (rc) {
+            destroy(b);
+            return rc;
+        }
     }
     *out = b;
     return KBWH_OK;
@@ -690,14 +1169,148 @@ int32_t kbwh_batch_stats(const kbwh_batch *b, uint64_t *gates, uint64_t *passes,
     if (!b || !gates || !passes || !amplitudes) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
     *gates = b->ngates;
     *passes = b->npasses;
+    std::vector<uint8_t> grouped(b->circs.size(), 0);
     uint64_t amps = 0;
-    for (const BCirc &c : b->circs) amps += (1ull << c.n_eff) * c.num_passes;
+    uint64_t np = 0;
+    for (const SetGroup &G : b->groups) {
+        for (const uint32_t c : G.circ) grouped[c] = 1;
+        const uint64_t per = G.passes.size() + G.sweeps.size();
+        amps += ((uint64_t)G.nsets << G.n_eff) * per;
+        np += per;
+    }
+    for (size_t c = 0; c < b->circs.size(); ++c)
+        if (!grouped[c]) {
+            amps += (1ull << b->circs[c].n_eff) * b->circs[c].num_passes;
+            np += b->circs[c].num_passes;
+        }
+    *passes = np;
     *amplitudes = amps;
     return KBWH_OK;
 }
 
 int32_t kbwh_batch_free(kbwh_batch *b) {
     destroy(b);
+    return KBWH_OK;
+}
+
+int32_t kbwh_batch_groups(const kbwh_batch *b, kbwh_batch_group_info *info, uint32_t capacity,
+                          uint32_t *num_groups) {
+    if (!b || !num_groups || (capacity && !info)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    *num_groups = (uint32_t)b->groups.size();
+    for (uint32_t g = 0; g < std::min<uint32_t>(capacity, (uint32_t)b->groups.size()); ++g) {
+        const SetGroup &G = b->groups[g];
+        kbwh_batch_group_info &I = info[g];
+        I.num_qubits = G.n_eff;
+        I.num_sets = G.nsets;
+        I.tile_targets = G.cap;
+        I.compiled = 0;
+        for (void *f : G.fns) I.compiled += f != nullptr;
+        I.passes = (uint32_t)G.passes.size();
+        I.sweeps = (uint32_t)G.sweeps.size();
+        I.tuned_ms = G.tuned_ms;
+    }
+    return KBWH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The graph holds the groups' tables and kernels: a new configuration
+// drops it, and the next run captures again.
+void drop_graph(kbwh_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int32_t configure_group(kbwh_batch *b, SetGroup &G, uint32_t cap, bool compiled) {
+    if (G.cap != cap || G.passes.empty()) {
+        int32_t rc = plan_group(G, cap, b->in_circuits.data(), b->in_ops.data(), b->in_paulis.data(),
+                                b->in_qubits.data(), b->in_offsets.data(), b->in_coeffs.data());
+        if (rc) return rc;
+    }
+    return group_kernels(G, b->device, compiled);
+}
+
+// one group's run, timed on the batch stream (after one untimed run)
+int32_t time_group(kbwh_batch *b, const SetGroup &G, double *ms) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = enqueue_group(G, b->states, b->d_params, b->d_results, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+    if (e == hipSuccess) e = enqueue_group(G, b->states, b->d_params, b->d_results, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    *ms = t;
+    return hip_status(e, "batch group timing");
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t kbwh_batch_configure(kbwh_batch *b, uint32_t tile_targets, int32_t compiled) {
+    if (!b) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null handle");
+    if (tile_targets < 1 || tile_targets > (uint32_t)(TL - 3))
+        return set_error(KBWH_ERR_INVALID_ARGUMENT, "tile targets must be 1..8");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_status(e, "batch sync");
+    drop_graph(b);
+    for (SetGroup &G : b->groups) {
+        int32_t rc = configure_group(b, G, tile_targets, compiled != 0);
+        if (rc) return rc;
+        G.tuned_ms = 0.0;
+    }
+    return KBWH_OK;
+}
+
+int32_t kbwh_batch_autotune(kbwh_batch *b, const double *params, double *ms_out, uint32_t capacity) {
+    if (!b || (b->nparams && !params) || (capacity && !ms_out)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_status(e, "batch sync");
+    drop_graph(b);
+    if (b->nparams) {
+        e = hipMemcpy(b->d_params, params, b->nparams * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_status(e, "batch tune parameters");
+    }
+    static const uint32_t caps[] = {TL - 3, 6, 4};
+    constexpr uint32_t kCand = 4;  // generic kernel at each cap, then compiled at the best cap
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        SetGroup &G = b->groups[g];
+        double best = 0.0;
+        uint32_t best_cap = TL - 3;
+        bool best_comp = false;
+        double t[kCand] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < 3; ++i) {
+            int32_t rc = configure_group(b, G, caps[i], false);
+            if (!rc) rc = time_group(b, G, &t[i]);
+            if (rc) return rc;
+            if (i == 0 || t[i] < best) {
+                best = t[i];
+                best_cap = caps[i];
+            }
+        }
+        int32_t rc = configure_group(b, G, best_cap, true);
+        bool any = false;
+        for (void *f : G.fns) any = any || f != nullptr;
+        if (!rc && any) rc = time_group(b, G, &t[3]);
+        if (rc) return rc;
+        if (any && t[3] < best) {
+            best = t[3];
+            best_comp = true;
+        }
+        rc = configure_group(b, G, best_cap, best_comp);
+        if (rc) return rc;
+        G.tuned_ms = best;
+        for (uint32_t i = 0; i < kCand && g * kCand + i < capacity; ++i) ms_out[g * kCand + i] = t[i];
+    }
     return KBWH_OK;
 }
 

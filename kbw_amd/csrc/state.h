@@ -115,12 +115,19 @@ hipError_t launch_tile_contracted(double2 *amp, const void *spec, const void *ga
 int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops,
                      const double *params, bool reorder = false, bool compiled = false);
 // Pass-specialised kernels (tile_jit.hip): compiles and loads on `device` the
-// kernel of each pass {host tile::TGate records, count} (cached by the
-// pass's structure); fns[i] is null where pass i runs the generic kernel.
-int32_t jit_prepare(int device, const std::vector<std::pair<const void *, uint32_t>> &passes,
-                    std::vector<void *> &fns);
+// kernel of each pass (cached by the pass's structure and flags); fns[i] is
+// null where pass i runs the generic kernel.
+constexpr uint32_t kJitInit = 1;  // the pass starts from |0...0> of each set instead of a load
+struct JitPass {
+    const void *gates;  // host tile::TGate records of the pass (one set)
+    uint32_t count;     // 0: no specialised kernel
+    uint32_t flags;
+};
+int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<void *> &fns);
+// set_shift < 32: per-set records (tile t takes gates + (t >> set_shift) * count);
+// set_mask: the amplitude mask of one set (kJitInit passes)
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
-                      hipStream_t stream);
+                      hipStream_t stream, uint32_t set_shift = 32, uint64_t set_mask = ~0ull);
 void free_adjoint(kbwh_state *s);
 // Tiled reverse sweep of the adjoint gradient (n >= 11): slots[j] receives the
 // term of the j-th parameterised op; partial holds kAdjTiledScratch doubles.

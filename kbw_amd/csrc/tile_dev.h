@@ -1,7 +1,7 @@
 // Device side of the tile machinery: the tile layout, the gate record and
 // the per-amplitude gate application in registers.  Self-contained (no
 // standard-library or host headers), so the same text also compiles at run
-// time into pass-specialised kernels (tile_jit.cpp).
+// time into pass-specialised kernels (tile_jit.hip).
 #pragma once
 
 #ifndef __HIPCC_RTC__
@@ -342,6 +342,33 @@ __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid
     }
 }
 
+// The wave-bit part of thread_pair on its own, for the compile-time dispatch
+// (pair_at): target on wave bit tp (6 <= tp < TB).
+template <int PK, bool CTRL>
+__device__ __forceinline__ void wave_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, uint32_t tp, const TGate &G) {
+    const uint32_t cm = G.cm;
+    const bool tok = thread_ctrl_ok(tid, cm);
+    const bool one = ((tid >> tp) & 1u) != 0;
+    const uint32_t pt = tid ^ (1u << tp);
+#pragma unroll
+    for (int q = 0; q < TR; q += kChunk) {
+        __syncthreads();  // earlier exchange reads are done
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) lds[tid + (r << TB)] = v[q + r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) {
+            const dv2 o = lds[pt + (r << TB)];
+            const dv2 out = pair_val_lane<PK>(v[q + r], o, one, one ? 0x80000000u : 0u, G);
+            if constexpr (CTRL) {
+                v[q + r] = slot_ctrl_ok(tok, q + r, cm) ? out : v[q + r];
+            } else {
+                v[q + r] = out;
+            }
+        }
+    }
+}
+
 template <int PK, bool CTRL>
 __device__ __forceinline__ void apply_pair_t(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
     const uint32_t tp = G.tpos;
@@ -525,6 +552,97 @@ __device__ __forceinline__ uint64_t slot_offset(uint64_t dt, int r, const TileSp
     return o;
 }
 
+// ---- compile-time dispatch (pass-specialised kernels) ----
+// The same paths as apply_pair / apply_diag, chosen from the record's
+// structural fields given as template arguments, so a kernel compiled for a
+// pass instantiates only the path each gate takes.  The coefficients still
+// come from the record.
+
+template <uint32_t TP, int PK, bool CTRL>
+__device__ __forceinline__ void pair_at(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
+    if constexpr (TP >= (uint32_t)TB) {
+        reg_pair<(int)(TP - TB), PK, CTRL>(v, tid, G);
+    } else if constexpr (TP < 6) {
+        constexpr int LX = (TP & 4u) ? LX_SHFL : (TP & 2u) ? ((TP & 1u) ? LX_R8 : LX_R4) : ((TP & 1u) ? LX_Q2 : LX_Q1);
+        lane_pair<PK, CTRL, LX>(v, tid, TP, G);
+    } else {
+        wave_pair<PK, CTRL>(v, lds, tid, TP, G);
+    }
+}
+
+// apply_diag_t with its runtime tests resolved: DK, target position TP, cm,
+// and the record's thread / register-slot masks DLO / DHI
+template <int DK, uint32_t TP, uint32_t CM, uint32_t DLO, uint32_t DHI>
+__device__ __forceinline__ void diag_at(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
+    const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
+    constexpr bool reg_t = TP >= (uint32_t)TB && TP != kOutside;
+    constexpr bool out_t = TP == kOutside;
+    constexpr bool thr_t = !reg_t && !out_t;
+    if constexpr (DK == DK_Z || DK == DK_P) {
+        const bool sel = (tid & DLO) == DLO;
+        if constexpr (DK == DK_Z) {
+            const uint32_t m = sel ? 0x80000000u : 0u;
+#pragma unroll
+            for (int r = 0; r < TR; ++r)
+                if (((uint32_t)r & DHI) == DHI) v[r] = flip_sign(v[r], m);
+        } else {
+            const dv2 f = sel ? ph1 : mk2(1.0, 0.0);
+#pragma unroll
+            for (int r = 0; r < TR; ++r)
+                if (((uint32_t)r & DHI) == DHI) v[r] = cmul(v[r], f);
+        }
+    } else {
+        const bool ok = (tid & DLO) == DLO;
+        if constexpr (!reg_t) {
+            bool one;
+            if constexpr (thr_t)
+                one = ((tid >> TP) & 1u) != 0;
+            else
+                one = (base & G.otb) != 0;
+            const dv2 f = ok ? (one ? ph1 : ph0) : mk2(1.0, 0.0);
+#pragma unroll
+            for (int r = 0; r < TR; ++r)
+                if (((uint32_t)r & DHI) == DHI) v[r] = cmul(v[r], f);
+        } else if constexpr (CM == 0) {
+            constexpr uint32_t TM = 1u << (TP - TB);
+#pragma unroll
+            for (int r = 0; r < TR; ++r) v[r] = cmul(v[r], ((uint32_t)r & TM) ? ph1 : ph0);
+        } else {
+            constexpr uint32_t rb = TP - TB;
+#pragma unroll
+            for (int r = 0; r < TR; ++r) {
+                const dv2 ph = ((uint32_t)r & DHI) == DHI ? ((((uint32_t)r >> rb) & 1u) ? ph1 : ph0) : mk2(1.0, 0.0);
+                v[r] = cmul(v[r], ok ? ph : mk2(1.0, 0.0));
+            }
+        }
+    }
+}
+
+// One gate whose kind, target position, in-tile controls and diagonal masks
+// are compile-time constants (the outside-control test stays with the caller).
+template <uint32_t KIND, uint32_t TP, uint32_t CM, uint32_t DLO, uint32_t DHI>
+__device__ __forceinline__ void gate_at(dv2 (&v)[TR], dv2 *lds, uint64_t base, uint32_t tid, const TGate &G) {
+    if constexpr (KIND == KZ) {
+        diag_at<DK_Z, TP, CM, DLO, DHI>(v, base, tid, G);
+    } else if constexpr (KIND == KP) {
+        diag_at<DK_P, TP, CM, DLO, DHI>(v, base, tid, G);
+    } else if constexpr (KIND == KRZ) {
+        diag_at<DK_RZ, TP, CM, DLO, DHI>(v, base, tid, G);
+    } else if constexpr (KIND == KX) {
+        pair_at<TP, PK_X, CM != 0>(v, lds, tid, G);
+    } else if constexpr (CM != 0) {
+        pair_at<TP, PK_GEN, true>(v, lds, tid, G);
+    } else if constexpr (KIND == KY) {
+        pair_at<TP, PK_Y, false>(v, lds, tid, G);
+    } else if constexpr (KIND == KH) {
+        pair_at<TP, PK_H, false>(v, lds, tid, G);
+    } else if constexpr (KIND == KRX) {
+        pair_at<TP, PK_RX, false>(v, lds, tid, G);
+    } else {
+        pair_at<TP, PK_TWO, false>(v, lds, tid, G);
+    }
+}
+
 // One gate of a pass on the tile of this workgroup (base: the tile's bits
 // outside the tile qubits).
 __device__ __forceinline__ void apply_gate(dv2 (&v)[TR], dv2 *lds, uint64_t base, uint32_t tid, const TGate &G) {
@@ -548,6 +666,13 @@ __device__ __forceinline__ uint64_t thread_offset(uint64_t base, uint32_t tid, c
 __device__ __forceinline__ void tile_load(dv2 (&v)[TR], const dv2 *__restrict__ a, uint64_t dt, const TileSpec &ts) {
 #pragma unroll
     for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
+}
+
+// |0...0> of every set in registers: amplitude 1 where the index within its
+// set (index & set_mask) is 0
+__device__ __forceinline__ void tile_zero_state(dv2 (&v)[TR], uint64_t dt, const TileSpec &ts, uint64_t set_mask) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) v[r] = mk2((slot_offset(dt, r, ts) & set_mask) == 0 ? 1.0 : 0.0, 0.0);
 }
 
 __device__ __forceinline__ void tile_store(const dv2 (&v)[TR], dv2 *__restrict__ a, uint64_t dt, const TileSpec &ts) {

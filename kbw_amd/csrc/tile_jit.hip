@@ -7,10 +7,12 @@
 // moves per gate), so a pass of 40+ gates is bound by instruction issue.  A
 // specialised kernel is the same load / gates / store, with each gate's
 // structural fields (kind, tile position, controls inside and outside the
-// tile, diagonal masks) written into the source as constants: the dispatch
-// folds away at compile time and the gates run as one straight line.  The
-// coefficients are still read from the gate records, so one kernel serves
-// every pass of the same structure whatever its angles.
+// tile, diagonal masks) given as template arguments of gate_at (tile_dev.h):
+// only the path each gate takes is instantiated (which also keeps the compile
+// short: folding the runtime dispatch away instead took 7x longer) and the
+// gates run as one straight line.  The coefficients are still read from the
+// gate records, so one kernel serves every pass of the same structure
+// whatever its angles.
 //
 // Kernels are compiled with hiprtc (loaded on first use; a library without it
 // runs the generic kernel) from the device tile header (tile_dev.h, embedded
@@ -80,10 +82,12 @@ const Rtc &rtc() {
     return r;
 }
 
-// The structure of a pass: the fields of each record that select code.
-std::string pass_key(const TGate *g, uint32_t count) {
+// The structure of a pass: the fields of each record that select code, and
+// the pass flags.
+std::string pass_key(const TGate *g, uint32_t count, uint32_t flags) {
     std::string k;
-    k.reserve(count * 40);
+    k.reserve(count * 56 + 4);
+    k.append((const char *)&flags, sizeof flags);
     for (uint32_t i = 0; i < count; ++i) {
         const uint64_t f[7] = {g[i].kind, g[i].tpos, g[i].cm, g[i].ocm, g[i].otb, g[i].dlo, g[i].dhi};
         k.append((const char *)f, sizeof f);
@@ -97,28 +101,39 @@ uint64_t fnv1a(const std::string &s) {
     return h;
 }
 
-std::string pass_source(const TGate *g, uint32_t count, const std::string &name) {
+// The kernel of one pass.  Arguments: the state, the pass's tile qubits, its
+// gate records, the tile count, set_shift (< 32: the state holds parameter
+// sets of 2^(set_shift + TL) amplitudes one after another, and tile t takes
+// the records gp + (t >> set_shift) * count) and, for an initialising pass
+// (JIT_INIT), the amplitude mask of one set: the tile starts as |0...0> of
+// its set in registers instead of being loaded.
+std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const std::string &name) {
     std::string s;
     s.reserve(1024 + count * 200);
     s += "#define KBWH_TILE_CONTRACT 1\n#include \"tile_dev.h\"\nusing namespace kbwh::tile;\n";
     s += "extern \"C\" __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, "
          "KBWH_TILE_WAVES))) void ";
     s += name;
-    s += "(dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gp, uint32_t ntiles) {\n"
+    s += "(dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gp, uint32_t ntiles, uint32_t set_shift,\n"
+         "    uint64_t set_mask) {\n"
          "  __shared__ dv2 lds[TT * kChunk];\n"
          "  const uint32_t tid = threadIdx.x, tile = blockIdx.x;\n"
-         "  if (tile >= ntiles) return;\n"
-         "  const uint64_t base = tile_base(tile, ts);\n"
+         "  if (tile >= ntiles) return;\n";
+    s += "  if (set_shift < 32) gp += (size_t)(tile >> set_shift) * " + std::to_string(count) + "u;\n";
+    s += "  const uint64_t base = tile_base(tile, ts);\n"
          "  const uint64_t dt = thread_offset(base, tid, ts);\n"
-         "  dv2 v[TR];\n"
-         "  tile_load(v, a, dt, ts);\n";
+         "  dv2 v[TR];\n";
+    s += (flags & kJitInit) ? "  tile_zero_state(v, dt, ts, set_mask);\n" : "  tile_load(v, a, dt, ts);\n";
     char line[320];
     for (uint32_t i = 0; i < count; ++i) {
         const TGate &G = g[i];
-        snprintf(line, sizeof line,
-                 "  { TGate G = gp[%u]; G.kind = %uu; G.tpos = %uu; G.cm = %uu; G.ocm = %lluull; G.otb = %lluull; "
-                 "G.dlo = %uu; G.dhi = %uu; apply_gate(v, lds, base, tid, G); }\n",
-                 i, G.kind, G.tpos, G.cm, (unsigned long long)G.ocm, (unsigned long long)G.otb, G.dlo, G.dhi);
+        const int k = snprintf(line, sizeof line, "  { const TGate &G = gp[%u]; ", i);
+        int m = k;
+        if (G.ocm)
+            m += snprintf(line + m, sizeof line - m, "if ((base & %lluull) == %lluull) ", (unsigned long long)G.ocm,
+                          (unsigned long long)G.ocm);
+        snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tid, G); }\n", G.kind,
+                 G.tpos, G.cm, G.dlo, G.dhi);
         s += line;
     }
     s += "  tile_store(v, a, dt, ts);\n}\n";
@@ -197,8 +212,7 @@ constexpr uint32_t kMaxSpecialisedGates = 256;
 // Compiles (in parallel) every pass of the list not compiled yet, loads the
 // kernels on `device` and returns one function per pass (null: run the
 // generic kernel for it).  passes[i] = {host records, count}.
-int32_t jit_prepare(int device, const std::vector<std::pair<const void *, uint32_t>> &passes,
-                    std::vector<void *> &fns) {
+int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<void *> &fns) {
     Cache &C = cache();
     fns.assign(passes.size(), nullptr);
     if (device < 0 || device >= kMaxDevices) return KBWH_OK;
@@ -208,13 +222,13 @@ int32_t jit_prepare(int device, const std::vector<std::pair<const void *, uint32
     std::vector<std::pair<std::shared_ptr<Kernel>, std::string>> todo;
     {
         for (size_t i = 0; i < passes.size(); ++i) {
-            const TGate *g = (const TGate *)passes[i].first;
-            const uint32_t cnt = passes[i].second;
+            const TGate *g = (const TGate *)passes[i].gates;
+            const uint32_t cnt = passes[i].count, fl = passes[i].flags;
             if (cnt == 0 || cnt > kMaxSpecialisedGates) {
                 C.generic++;
                 continue;
             }
-            std::string key = pass_key(g, cnt);
+            std::string key = pass_key(g, cnt, fl);
             auto it = C.kernels.find(key);
             if (it != C.kernels.end()) {
                 ks[i] = it->second;
@@ -227,7 +241,7 @@ int32_t jit_prepare(int device, const std::vector<std::pair<const void *, uint32
             k->name = nm;
             C.kernels.emplace(key, k);
             ks[i] = k;
-            todo.push_back({k, pass_source(g, cnt, k->name)});
+            todo.push_back({k, pass_source(g, cnt, fl, k->name)});
         }
     }
     if (!todo.empty()) {
@@ -273,12 +287,12 @@ int32_t jit_prepare(int device, const std::vector<std::pair<const void *, uint32
 }
 
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
-                      hipStream_t stream) {
+                      hipStream_t stream, uint32_t set_shift, uint64_t set_mask) {
     TileSpec ts;
     memcpy(&ts, spec, sizeof ts);
     dv2 *a = (dv2 *)amp;
     const TGate *g = (const TGate *)gates;
-    void *args[] = {&a, &ts, &g, &ntiles};
+    void *args[] = {&a, &ts, &g, &ntiles, &set_shift, &set_mask};
     return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
 }
 
@@ -324,7 +338,7 @@ extern "C" int32_t kbwh_compiled_pass_code(const kbwh_op *ops, uint64_t num_ops,
     for (const PhysOp &o : cur.ops) g.push_back(make_tgate(o, pos_of, num_qubits));
     std::vector<char> obj;
     std::string log;
-    if (!compile_source(pass_source(g.data(), (uint32_t)g.size(), "kbwh_pass"), obj, log))
+    if (!compile_source(pass_source(g.data(), (uint32_t)g.size(), 0, "kbwh_pass"), obj, log))
         return set_error(KBWH_ERR_EXECUTION_FAILED, "pass kernel did not compile: " + log.substr(0, 400));
     if (code) {
         if (*size < obj.size()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "buffer too small");

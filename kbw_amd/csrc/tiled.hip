@@ -451,10 +451,10 @@ struct PassBuilder {
         }
         std::vector<void *> fns;
         if (compiled) {
-            std::vector<std::pair<const void *, uint32_t>> lst;
+            std::vector<JitPass> lst;
             lst.reserve(launches.size());
             for (const Launch &L : launches)
-                lst.push_back({L.perm ? nullptr : (const void *)&gates[L.first], L.perm ? 0u : L.count});
+                lst.push_back(JitPass{L.perm ? nullptr : (const void *)&gates[L.first], L.perm ? 0u : L.count, 0u});
             const int32_t rc = jit_prepare(s->device, lst, fns);
             if (rc) return rc;
         }

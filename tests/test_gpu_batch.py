@@ -105,3 +105,86 @@ def test_batch_c3_workload_spot_check():
         e = s.exp_value(ham)
         s.close()
         assert abs(out[i] - e) <= 1e-12 * max(1.0, abs(e)), (i, n, out[i], e)
+
+
+def group_batch(seed, specs):
+    """HEA circuits: specs = [(n, layers, count)], each with its own angles and
+    transverse-field weights (one structure per (n, layers)), interleaved
+    with a few random-gate circuits that run in the persistent kernel."""
+    rng = np.random.default_rng(seed)
+    circs, params = [], []
+    for n, layers, count in specs:
+        for _ in range(count):
+            ops, p = circuits.hea(n, layers, seed=int(rng.integers(1 << 30)))
+            ham = circuits.tfim(n, seed=int(rng.integers(1 << 30)))
+            circs.append((n, ops, ham, len(p)))
+            params.append(p)
+    for n in (12, 15):
+        ops = circuits.random_circuit(rng, n, 60, max_controls=2)
+        rot = np.isin(ops["kind"], [4, 5, 6, 7])
+        ops["param"][rot] = np.arange(rot.sum())
+        circs.append((n, ops, circuits.random_hamiltonian(rng, n, 8, max_weight=3), int(rot.sum())))
+        params.append(ops["angle"][rot].copy())
+    order = rng.permutation(len(circs))
+    circs = [circs[i] for i in order]
+    params = [params[i] for i in order]
+    return circs, np.concatenate(params)
+
+
+def check_against_oracle(b, circs, params, out, picks, tol=1e-11):
+    off = b.param_offsets
+    for i in picks:
+        n, ops, ham, _ = circs[i]
+        e, st = oracle_expval(n, ops, params[off[i]:off[i + 1]], ham)
+        assert abs(out[i] - e) <= tol * max(1.0, abs(e)), (i, n, out[i], e)
+        assert np.max(np.abs(b.state(i, 1 << n) - st)) < 1e-12, (i, n)
+
+
+def test_batch_structure_groups_match_oracle():
+    """Two groups (40 x 14 qubits = 320 tiles, 300 x 10 qubits on 11-qubit
+    sets = 300 tiles), a group too small to split off (5 x 13 qubits) and two
+    random circuits: energies and states against the oracle."""
+    print("batch groups", flush=True)
+    circs, params = group_batch(31, [(14, 3, 40), (10, 2, 300), (13, 2, 5)])
+    b = Batch(circs)
+    groups = b.groups()
+    assert sorted((g["num_qubits"], g["num_sets"]) for g in groups) == [(11, 300), (14, 40)]
+    out = b.run(params)
+    rng = np.random.default_rng(7)
+    nq = np.array([c[0] for c in circs])
+    picks = list(rng.choice(np.nonzero(nq == 14)[0], 4, replace=False))
+    picks += list(rng.choice(np.nonzero(nq == 10)[0], 4, replace=False))
+    picks += list(np.nonzero((nq == 13) | (nq == 12) | (nq == 15))[0])
+    check_against_oracle(b, circs, params, out, picks)
+    b.close()
+
+
+def test_batch_groups_configure_autotune_and_graph():
+    """Target caps and compiled kernels per group agree with the generic run
+    to rounding; the tuner keeps one configuration per group and reports its
+    time; graph replays of the tuned batch equal its eager runs."""
+    print("batch groups tuning", flush=True)
+    circs, params = group_batch(32, [(16, 3, 12), (12, 3, 200)])
+    b = Batch(circs)
+    assert len(b.groups()) == 2
+    base = b.run(params)
+    b.configure(6, compiled=True)
+    assert all(g["compiled"] == g["passes"] for g in b.groups())
+    assert np.max(np.abs(b.run(params) - base)) <= 1e-12
+    ms = b.autotune(params)
+    assert ms.shape == (2, 4) and np.all(ms[:, :3] > 0)
+    info = b.groups()
+    assert all(g["tuned_ms"] > 0 and g["tile_targets"] in (8, 6, 4) for g in info)
+    tuned = b.run(params)
+    assert np.max(np.abs(tuned - base)) <= 1e-12
+    rng = np.random.default_rng(8)
+    check_against_oracle(b, circs, params, tuned, rng.choice(len(circs), 6, replace=False))
+    p2 = params + rng.normal(0, 0.2, size=params.shape)
+    eager = b.run(p2)
+    b.use_graph(True)
+    for rep in range(3):
+        assert np.array_equal(b.run(p2), eager), rep
+        b.use_graph(False)
+        assert np.array_equal(b.run(params), tuned)
+        b.use_graph(True)
+    b.close()

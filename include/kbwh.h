@@ -325,6 +325,13 @@ int32_t kbwh_batch_configure(kbwh_batch *b, uint32_t tile_targets, int32_t compi
 int32_t kbwh_batch_autotune(kbwh_batch *b, const double *params, double *ms, uint32_t capacity);
 int32_t kbwh_batch_free(kbwh_batch *b);
 
+/* Fault injection (tests; no counterpart in the reference): after `after`
+ * more device allocations by this library (any handle, any thread), the
+ * next one fails as if the device were out of memory (the call that made it
+ * returns KBWH_ERR_OUT_OF_MEMORY), and allocation works again afterwards.
+ * A negative value cancels a pending failure. */
+int32_t kbwh_inject_alloc_failure(int64_t after);
+
 /* ---- raw access (tests, tooling) ------------------------------------- */
 /* Copies amplitudes [first, first+count) of the physical buffer. */
 int32_t kbwh_read_amplitudes(kbwh_state *s, uint64_t first, uint64_t count, double *out);

@@ -8,7 +8,9 @@ import os
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkbwh.so")
+# KBWH_LIB: another build of the library (the sanitized host-only build of
+# the CPU tests); default the in-tree build
+LIB_PATH = os.environ.get("KBWH_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkbwh.so")
 
 OK = 0
 ERR_EXECUTION_FAILED = 1
@@ -84,6 +86,7 @@ SIGNATURES = {
     "kbwh_batch_state": [_vp, _u32, _u64, _p],
     "kbwh_batch_stats": [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)],
     "kbwh_batch_groups": [_vp, _p, _u32, C.POINTER(_u32)],
+    "kbwh_inject_alloc_failure": [C.c_int64],
     "kbwh_batch_configure": [_vp, _u32, _i32],
     "kbwh_batch_autotune": [_vp, _p, _p, _u32],
     "kbwh_batch_free": [_vp],

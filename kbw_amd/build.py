@@ -106,6 +106,65 @@ def _wait(item, verbose):
         sys.stderr.write(out)
 
 
+SAN_DIR = os.path.join(ROOT, "_build", "asan")
+SAN_LIB = os.path.join(SAN_DIR, "libkbwh.so")
+CLANGXX = "/opt/rocm/llvm/bin/clang++"
+
+
+def asan_runtime():
+    """The clang AddressSanitizer runtime the sanitized build links to (to
+    preload into a process that loads it), or None."""
+    try:
+        out = subprocess.run([CLANGXX, "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True,
+                             text=True).stdout.strip()
+    except OSError:
+        return None
+    if out and os.path.isabs(out) and os.path.exists(out):
+        return out
+    import glob
+
+    hits = glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so")
+    return hits[0] if hits else None
+
+
+def build_sanitized(jobs=8, verbose=False):
+    """A build of the library with AddressSanitizer and UBSan on its host
+    code (planners, plan checks, argument checks, the C ABI; device code is
+    built as usual, uninstrumented) for the CPU tests.  Writes
+    _build/asan/libkbwh.so."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+    host_flags = [f for f in HIP_FLAGS if not f.startswith("--offload-arch")]
+    procs, objs = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(SAN_DIR, src.replace(".hip", ".o"))
+        objs.append(obj)
+        procs.append([HIPCC, f"--offload-arch={ARCH}", *host_flags, *HIP_SOURCE_FLAGS.get(src, []), "-g", *san,
+                      "-c", os.path.join(CSRC, src), "-o", obj])
+    cflags = ["-O1", "-g", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+              "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer"]
+    for src in CXX_SOURCES:
+        obj = os.path.join(SAN_DIR, src.replace(".cpp", ".o"))
+        objs.append(obj)
+        procs.append([CLANGXX, *cflags, "-c", os.path.join(CSRC, src), "-o", obj])
+    emb = embed_header(os.path.join(CSRC, "tile_dev.h"), "kTileDevSource", os.path.join(SAN_DIR, "tile_dev_src.cpp"))
+    objs.append(emb[:-4] + ".o")
+    procs.append([CLANGXX, *cflags, "-c", emb, "-o", objs[-1]])
+    running = []
+    for cmd in procs:
+        running.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        if len(running) >= jobs:
+            _wait(running.pop(0), verbose)
+    while running:
+        _wait(running.pop(0), verbose)
+    tmp = SAN_LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address", "-Xarch_host",
+          "-fsanitize=undefined", "-shared-libsan", "-o", tmp, *objs, "-ldl"])
+    os.replace(tmp, SAN_LIB)
+    return SAN_LIB
+
+
 def build_oracle():
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], check=True)

@@ -193,7 +193,7 @@ inline uint32_t grid_for(uint64_t items, uint64_t per) { return (uint32_t)((item
 // reverse sweep takes the pair as one state of n + 1 qubits)
 int32_t ensure_vecs(kbwh_state *s) {
     if (s->adj_phi) return KBWH_OK;
-    const int32_t rc = hip_status(hipMalloc(&s->adj_phi, 2 * s->size * sizeof(double2)), "adjoint vectors");
+    const int32_t rc = hip_status(dev_alloc(&s->adj_phi, 2 * s->size * sizeof(double2)), "adjoint vectors");
     if (rc) {
         s->adj_phi = nullptr;
         return rc;
@@ -264,7 +264,7 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
         }
         s->adj_partial = nullptr;
         s->adj_partial_cap = 0;
-        rc = hip_status(hipMalloc(&s->adj_partial, need * sizeof(double)), "adjoint scratch");
+        rc = hip_status(dev_alloc(&s->adj_partial, need * sizeof(double)), "adjoint scratch");
         if (rc) return rc;
         s->adj_partial_cap = need;
     }

@@ -359,7 +359,7 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
         if (e == hipSuccess && s->tile_gates) e = hipFree(s->tile_gates);
         s->tile_gates = nullptr;
         s->tile_gates_cap = 0;
-        if (e == hipSuccess) e = hipMalloc(&s->tile_gates, bytes);
+        if (e == hipSuccess) e = dev_alloc(&s->tile_gates, bytes);
         if (e != hipSuccess) return hip_status(e, "adjoint gate table");
         s->tile_gates_cap = bytes;
     }

@@ -82,7 +82,7 @@ int32_t kbwh_autotune(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const
     const uint64_t keep_passes = s->last_passes, keep_gates = s->last_gates;
     double2 *buf = nullptr;
     hipError_t e = hipStreamSynchronize(s->stream);
-    if (e == hipSuccess) e = hipMalloc(&buf, s->size * sizeof(double2));
+    if (e == hipSuccess) e = dev_alloc(&buf, s->size * sizeof(double2));
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
         (void)hipGetLastError();
         return set_error(KBWH_ERR_OUT_OF_MEMORY, "no device memory for the tuning buffer");

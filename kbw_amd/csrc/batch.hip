@@ -525,7 +525,7 @@ void destroy(kbwh_batch *b) {
 template <typename T>
 hipError_t upload(T **dst, const std::vector<T> &src) {
     const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
-    hipError_t e = hipMalloc(dst, bytes);
+    hipError_t e = dev_alloc(dst, bytes);
     if (e == hipSuccess && !src.empty()) e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
     return e;
 }
@@ -884,17 +884,17 @@ int32_t plan_group(SetGroup &G, uint32_t cap, const kbwh_circuit *circuits, cons
     G.d_meta = nullptr;
     G.d_str = nullptr;
     G.d_coef = G.d_partial = nullptr;
-    hipError_t e = hipMalloc(&G.d_tmpl, std::max<size_t>(nrec, 1) * sizeof(TGate));
+    hipError_t e = dev_alloc(&G.d_tmpl, std::max<size_t>(nrec, 1) * sizeof(TGate));
     if (e == hipSuccess && nrec) e = hipMemcpy(G.d_tmpl, G.tmpl.data(), nrec * sizeof(TGate), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&G.d_recs, std::max<size_t>((size_t)nrec * S, 1) * sizeof(TGate));
-    if (e == hipSuccess) e = hipMalloc(&G.d_meta, meta.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = dev_alloc(&G.d_recs, std::max<size_t>((size_t)nrec * S, 1) * sizeof(TGate));
+    if (e == hipSuccess) e = dev_alloc(&G.d_meta, meta.size() * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemcpy(G.d_meta, meta.data(), meta.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&G.d_str, std::max<size_t>(strs.size(), 1) * sizeof(BString));
+    if (e == hipSuccess) e = dev_alloc(&G.d_str, std::max<size_t>(strs.size(), 1) * sizeof(BString));
     if (e == hipSuccess && !strs.empty())
         e = hipMemcpy(G.d_str, strs.data(), strs.size() * sizeof(BString), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&G.d_coef, coef.size() * sizeof(double));
+    if (e == hipSuccess) e = dev_alloc(&G.d_coef, coef.size() * sizeof(double));
     if (e == hipSuccess) e = hipMemcpy(G.d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&G.d_partial, std::max<uint64_t>(ntiles * G.sweeps.size(), 1) * sizeof(double));
+    if (e == hipSuccess) e = dev_alloc(&G.d_partial, std::max<uint64_t>(ntiles * G.sweeps.size(), 1) * sizeof(double));
     G.fns.assign(G.passes.size(), nullptr);
     return hip_status(e, "batch group tables");
 }
@@ -1070,17 +1070,17 @@ int32_t kbwh_batch_new(const kbwh_circuit *circuits, uint32_t num_circuits, cons
     b->nitems = (uint32_t)items.size();
     b->grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(items.size(), resident));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&b->states, std::max<uint64_t>(state_total, 1) * sizeof(dv2));
+    if (e == hipSuccess) e = dev_alloc(&b->states, std::max<uint64_t>(state_total, 1) * sizeof(dv2));
     if (e == hipSuccess) e = upload(&b->d_circs, circs);
     if (e == hipSuccess) e = upload(&b->d_items, items);
-    if (e == hipSuccess) e = hipMalloc(&b->d_partial, std::max<size_t>(items.size(), 1) * 8);
+    if (e == hipSuccess) e = dev_alloc(&b->d_partial, std::max<size_t>(items.size(), 1) * 8);
     if (e == hipSuccess) e = upload(&b->d_passes, passes);
     if (e == hipSuccess) e = upload(&b->d_tmpl, tmpl);
-    if (e == hipSuccess) e = hipMalloc(&b->d_live, std::max<size_t>(tmpl.size(), 1) * sizeof(TGate));
+    if (e == hipSuccess) e = dev_alloc(&b->d_live, std::max<size_t>(tmpl.size(), 1) * sizeof(TGate));
     if (e == hipSuccess) e = upload(&b->d_strings, strings);
-    if (e == hipSuccess) e = hipMalloc(&b->d_params, std::max<uint64_t>(num_params, 1) * 8);
-    if (e == hipSuccess) e = hipMalloc(&b->d_results, std::max<uint32_t>(num_circuits, 1) * 8);
-    if (e == hipSuccess) e = hipMalloc(&b->d_ctl, (2 + 2 * (size_t)num_circuits) * sizeof(uint32_t));
+    if (e == hipSuccess) e = dev_alloc(&b->d_params, std::max<uint64_t>(num_params, 1) * 8);
+    if (e == hipSuccess) e = dev_alloc(&b->d_results, std::max<uint32_t>(num_circuits, 1) * 8);
+    if (e == hipSuccess) e = dev_alloc(&b->d_ctl, (2 + 2 * (size_t)num_circuits) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(&b->h_params, std::max<uint64_t>(num_params, 1) * 8, 0);
     if (e == hipSuccess) e = hipHostMalloc(&b->h_results, ((size_t)num_circuits + 1) * 8, 0);
     if (e != hipSuccess) {

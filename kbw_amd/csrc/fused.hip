@@ -337,8 +337,8 @@ int32_t run_fused(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_
     hipError_t e = hipStreamSynchronize(s->stream);
     FBlock *dblocks = nullptr;
     double *damat = nullptr;
-    if (e == hipSuccess && !fblocks.empty()) e = hipMalloc(&dblocks, fblocks.size() * sizeof(FBlock));
-    if (e == hipSuccess && !amat.empty()) e = hipMalloc(&damat, amat.size() * sizeof(double));
+    if (e == hipSuccess && !fblocks.empty()) e = dev_alloc(&dblocks, fblocks.size() * sizeof(FBlock));
+    if (e == hipSuccess && !amat.empty()) e = dev_alloc(&damat, amat.size() * sizeof(double));
     if (e == hipSuccess && dblocks) e = hipMemcpy(dblocks, fblocks.data(), fblocks.size() * sizeof(FBlock), hipMemcpyHostToDevice);
     if (e == hipSuccess && damat) e = hipMemcpy(damat, amat.data(), amat.size() * sizeof(double), hipMemcpyHostToDevice);
     const uint32_t ntiles = (uint32_t)(1ull << (n - TL));

@@ -168,7 +168,7 @@ int32_t flush_queue(kbwh_state *s) {
         if (e == hipSuccess && s->tile_gates) e = hipFree(s->tile_gates);
         s->tile_gates = nullptr;
         s->tile_gates_cap = 0;
-        if (e == hipSuccess) e = hipMalloc(&s->tile_gates, bytes);
+        if (e == hipSuccess) e = dev_alloc(&s->tile_gates, bytes);
         if (e != hipSuccess) return hip_status(e, "queue buffer");
         s->tile_gates_cap = bytes;
     }

@@ -248,7 +248,7 @@ int32_t kbwh_dump(kbwh_state *s, const uint32_t *qubits, uint32_t nq, uint64_t c
     if (total == 0) return KBWH_OK;
     void *d_out = nullptr;
     e = hipMemcpyAsync(d_offsets, offsets.data(), nseg * 8, hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess) e = hipMalloc(&d_out, total * 24);
+    if (e == hipSuccess) e = dev_alloc(&d_out, total * 24);
     if (e != hipSuccess) return hip_status(e, "dump alloc");
     uint64_t *d_keys = (uint64_t *)d_out;
     double *d_re = (double *)(d_keys + total), *d_im = d_re + total;

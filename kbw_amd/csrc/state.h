@@ -87,6 +87,15 @@ struct kbwh_state {
 namespace kbwh {
 
 int32_t set_error(int32_t code, const std::string &msg);
+// Every device allocation of the library goes through dev_alloc: hipMalloc,
+// unless an injected failure (kbwh_inject_alloc_failure) is due, in which
+// case it returns hipErrorOutOfMemory without allocating.
+hipError_t dev_alloc_raw(void **p, size_t bytes);
+void inject_alloc_failure(int64_t after);
+template <typename T>
+inline hipError_t dev_alloc(T **p, size_t bytes) {
+    return dev_alloc_raw((void **)p, bytes);
+}
 int32_t hip_status(hipError_t e, const char *what);
 int32_t ensure_scratch(kbwh_state *s, size_t doubles);
 int32_t local_qubit(kbwh_state *s, uint32_t logical, uint32_t *out);

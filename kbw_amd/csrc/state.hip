@@ -10,6 +10,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -54,6 +56,30 @@ namespace kbwh {
 
 int32_t set_error(int32_t code, const std::string &msg) { return fail(code, msg); }
 
+// injected allocation failures: allocations left before the next one fails
+// (negative: none pending)
+static std::atomic<int64_t> g_fail_after{-1};
+
+void inject_alloc_failure(int64_t after) { g_fail_after.store(after < 0 ? -1 : after); }
+
+hipError_t dev_alloc_raw(void **p, size_t bytes) {
+    int64_t left = g_fail_after.load();
+    while (left >= 0) {
+        if (g_fail_after.compare_exchange_weak(left, left - 1)) {
+            if (left == 0) {
+                *p = nullptr;
+                return hipErrorOutOfMemory;
+            }
+            break;
+        }
+    }
+    const hipError_t e = hipMalloc(p, bytes);
+    // the failure is returned here; clear it from the runtime's last-error
+    // slot, or the next launch check (hipGetLastError) would report it again
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
+}
+
 int32_t hip_status(hipError_t e, const char *what) {
     if (e == hipSuccess) return KBWH_OK;
     return fail(e == hipErrorOutOfMemory ? KBWH_ERR_OUT_OF_MEMORY : KBWH_ERR_DEVICE,
@@ -68,7 +94,7 @@ int32_t grow_device(kbwh_state *s, void **buf, size_t *cap, size_t bytes) {
     }
     *buf = nullptr;
     *cap = 0;
-    HIPCHK(hipMalloc(buf, bytes));
+    HIPCHK(dev_alloc(buf, bytes));
     *cap = bytes;
     return KBWH_OK;
 }
@@ -78,7 +104,7 @@ int32_t ensure_scratch(kbwh_state *s, size_t doubles) {
     if (s->scratch) HIPCHK(hipFree(s->scratch));
     s->scratch = nullptr;
     s->scratch_cap = 0;
-    HIPCHK(hipMalloc(&s->scratch, doubles * sizeof(double)));
+    HIPCHK(dev_alloc(&s->scratch, doubles * sizeof(double)));
     s->scratch_cap = doubles;
     return KBWH_OK;
 }
@@ -226,9 +252,9 @@ int32_t kbwh_new(uint32_t num_qubits, uint32_t flags, uint32_t block_size, kbwh_
     s->size = 1ull << num_qubits;
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&s->amp, s->size * sizeof(double2));
-    if (e == hipSuccess) e = hipMalloc(&s->partial, (4096 + 64) * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&s->result, kResultSlots * sizeof(double));
+    if (e == hipSuccess) e = dev_alloc(&s->amp, s->size * sizeof(double2));
+    if (e == hipSuccess) e = dev_alloc(&s->partial, (4096 + 64) * sizeof(double));
+    if (e == hipSuccess) e = dev_alloc(&s->result, kResultSlots * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc(&s->host_result, kResultSlots * sizeof(double), 0);
     if (e == hipSuccess) e = launch_init(s->amp, s->size, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
@@ -292,6 +318,11 @@ int32_t kbwh_gate_mask(kbwh_state *s, int32_t kind, double angle, uint32_t targe
                        uint64_t ctrl_mask) {
     if (!s) return fail(KBWH_ERR_INVALID_ARGUMENT, "null handle");
     return s->use_queue ? queue_gate(s, kind, angle, target, ctrl_mask) : apply_gate(s, kind, angle, target, ctrl_mask);
+}
+
+int32_t kbwh_inject_alloc_failure(int64_t after) {
+    kbwh::inject_alloc_failure(after);
+    return KBWH_OK;
 }
 
 int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,

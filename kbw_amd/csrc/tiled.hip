@@ -237,7 +237,7 @@ int32_t upload_strings(kbwh_state *s, const std::vector<SString> &strings) {
         if (e == hipSuccess && s->tile_gates) e = hipFree(s->tile_gates);
         s->tile_gates = nullptr;
         s->tile_gates_cap = 0;
-        if (e == hipSuccess) e = hipMalloc(&s->tile_gates, bytes);
+        if (e == hipSuccess) e = dev_alloc(&s->tile_gates, bytes);
         if (e != hipSuccess) return hip_status(e, "string table");
         s->tile_gates_cap = bytes;
     }
@@ -445,7 +445,7 @@ struct PassBuilder {
             if (e == hipSuccess && s->tile_gates) e = hipFree(s->tile_gates);
             s->tile_gates = nullptr;
             s->tile_gates_cap = 0;
-            if (e == hipSuccess) e = hipMalloc(&s->tile_gates, bytes);
+            if (e == hipSuccess) e = dev_alloc(&s->tile_gates, bytes);
             if (e != hipSuccess) return hip_status(e, "tile gate buffer");
             s->tile_gates_cap = bytes;
         }

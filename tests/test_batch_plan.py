@@ -91,3 +91,25 @@ def test_c3_plan_is_stable():
     # every circuit has at least one pass; the buffer holds one slice per circuit
     assert p >= len(circs)
     assert a == sum(1 << max(c[0], TILE) for c in circs)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_batches_plan(seed):
+    """Random batches (1..26 qubits, random gates with parameters, random
+    Hamiltonians, gate-free circuits) plan without error, with at least one
+    pass per circuit and 2^max(n, 11) amplitudes per circuit pass."""
+    import numpy as np
+
+    rng = np.random.default_rng(300 + seed)
+    circs = []
+    for _ in range(int(rng.integers(1, 12))):
+        n = int(rng.integers(1, 27))
+        ops = circuits.random_circuit(rng, n, int(rng.integers(0, 120)), max_controls=min(3, n - 1)) if n > 1 \
+            else empty_ops()
+        rot = np.isin(ops["kind"], [4, 5, 6, 7])
+        ops["param"][rot] = np.arange(rot.sum())
+        ham = circuits.random_hamiltonian(rng, n, int(rng.integers(1, 10)), max_weight=min(3, n))
+        circs.append((n, ops, ham, int(rot.sum())))
+    gates, passes, amps = plan(circs)
+    assert passes >= len(circs)
+    assert amps >= sum(2 ** max(c[0], TILE) for c in circs)

@@ -107,3 +107,15 @@ def test_plan_rejects_bad_input():
         reorder_plan(ops, 5)
     with pytest.raises(KbwhError):
         reorder_plan(ops, 8, 0)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_plans_many_shapes(seed):
+    """Random circuits of 3..40 qubits with up to 5 controls per gate and
+    every target cap: the plan invariants hold (also run under the
+    AddressSanitizer / UBSan build of the planners)."""
+    rng = np.random.default_rng(900 + seed)
+    n = int(rng.integers(3, 41))
+    ops = circuits.random_circuit(rng, n, int(rng.integers(1, 400)), max_controls=min(5, n - 1))
+    for cap in (1, 3, 8, 64):
+        check_plan(ops, n, cap)

@@ -201,6 +201,27 @@ def hea_batch(num=4096, seed=12, n_low=10, n_high=20, layers=8):
     return circuits, np.concatenate(params)
 
 
+def every_position(kind, n=13, params=False):
+    """Tests: one gate of `kind` on each of qubits 0..10 (every position of an
+    11-qubit tile: a pass's planner places all eleven), each uncontrolled, with
+    a control on qubit 0 or 1, on qubit 3 or 4 and on qubit n - 1, with angles
+    of every sign and magnitude form; params=True gives each rotation its own
+    parameter (returns (ops, params))."""
+    ops = OpList()
+    vals = []
+    for q in range(11):
+        lane_c = [0] if q != 0 else [1]
+        tile_c = [3] if q != 3 else [4]
+        for ctrl, ang in zip(([], lane_c, tile_c, [n - 1]), (0.9, 2.5, 4.0, 5.9)):
+            a = ang + 0.01 * q
+            if params and kind in (P, RX, RY, RZ):
+                ops.add(kind, q, ctrl, angle=a, param=len(vals))
+                vals.append(a)
+            else:
+                ops.add(kind, q, ctrl, angle=a)
+    return (ops.array(), np.array(vals)) if params else ops.array()
+
+
 def random_circuit(rng, n, ngates, max_controls=3):
     """Random gates of every kind with random targets and controls (tests)."""
     ops = OpList()

@@ -197,7 +197,12 @@ __global__ __launch_bounds__(256) void k_sum_slots(const double *__restrict__ pa
     const uint32_t s = blockIdx.x, set = blockIdx.y;
     const double *p = partial + (size_t)s * grid + (size_t)set * wps;
     double a = 0.0;
-    for (uint32_t i = threadIdx.x; i < wps; i += 256) a += p[i];
+    // a loop with the same trip count in every lane (units built with
+    // uniform regions unstructurized keep per-lane loop exits out)
+    for (uint32_t i0 = 0; i0 < wps; i0 += 256) {
+        const uint32_t i = i0 + threadIdx.x;
+        if (i < wps) a += p[i];
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) a += __shfl_xor(a, off, 64);
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = a;

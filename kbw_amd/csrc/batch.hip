@@ -125,7 +125,11 @@ __global__ __launch_bounds__(256) void k_coef(const TGate *__restrict__ tmpl, TG
 // can be captured in a hipGraph): [0] item counter, [1] error flag, then a
 // barrier counter and a done counter per circuit.
 __global__ __launch_bounds__(256) void k_reset(uint32_t *ctl, uint32_t count) {
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += gridDim.x * 256) ctl[i] = 0;
+    // the same trip count in every lane (no per-lane loop exit in this unit)
+    for (uint32_t i0 = blockIdx.x * 256; i0 < count; i0 += gridDim.x * 256) {
+        const uint32_t i = i0 + threadIdx.x;
+        if (i < count) ctl[i] = 0;
+    }
 }
 
 // All chunks of a circuit have finished pass p before any starts pass p + 1:
@@ -400,7 +404,10 @@ __global__ __launch_bounds__(256) void k_gsum(const double *__restrict__ partial
     double acc = 0.0;
     for (uint32_t g = 0; g < nsweeps; ++g) {
         const double *p = partial + (size_t)g * ntiles + (size_t)s * tiles_per_set;
-        for (uint32_t t = threadIdx.x; t < tiles_per_set; t += 256) acc += p[t];
+        for (uint32_t t0 = 0; t0 < tiles_per_set; t0 += 256) {
+            const uint32_t t = t0 + threadIdx.x;
+            if (t < tiles_per_set) acc += p[t];
+        }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);

@@ -125,3 +125,35 @@ def test_torch_autograd_function():
     _, g = sim.adjoint_gradient(ops, params, ham)
     assert np.allclose(p.grad.numpy(), 2.0 * g, rtol=0, atol=1e-15)
     assert float(e) == pytest.approx(oracle_energy(n, ops, params, ham), abs=1e-12)
+
+
+@pytest.mark.parametrize("kind", [4, 5, 6, 7])
+def test_adjoint_every_position_and_control(kind):
+    """Parameterised P / RX / RY / RZ on each of qubits 0..10 of a 13-qubit
+    state (every tile position of the reverse sweep) under every control
+    placement, after H on every qubit and with X, Y, H between: gradients
+    against central differences of the oracle's energy (step 1e-6)."""
+    print(f"adjoint every position kind={kind}", flush=True)
+    n = 13
+    ops_k, params = circuits.every_position(kind, n, params=True)
+    pre = circuits.OpList()
+    for q in range(n):
+        pre.add(3, q)
+    mid = circuits.OpList()
+    for q in range(0, 11, 2):
+        mid.add((0, 1, 3)[q % 3], q, [(q + 5) % n])
+    ops = np.concatenate([pre.array(), ops_k[:22], mid.array(), ops_k[22:]])
+    rng = np.random.default_rng(60 + kind)
+    ham = circuits.random_hamiltonian(rng, n, 12, max_weight=3)
+    sim = DenseHip(n)
+    e, g = sim.adjoint_gradient(ops, params, ham)
+    assert e == pytest.approx(oracle_energy(n, ops, params, ham), abs=1e-12)
+    h = 1e-6
+    for j in range(len(params)):
+        p = params.copy()
+        p[j] += h
+        fp = oracle_energy(n, ops, p, ham)
+        p[j] -= 2 * h
+        fm = oracle_energy(n, ops, p, ham)
+        assert abs(g[j] - (fp - fm) / (2 * h)) <= 2e-8, (j, g[j], (fp - fm) / (2 * h))
+    sim.close()

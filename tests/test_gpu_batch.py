@@ -188,3 +188,37 @@ def test_batch_groups_configure_autotune_and_graph():
         assert np.array_equal(b.run(params), tuned)
         b.use_graph(True)
     b.close()
+
+
+@pytest.mark.parametrize("kind", range(8))
+def test_batch_every_kind_every_position(kind):
+    """One circuit of `kind` on each of qubits 0..10 of 13 under every
+    control placement, after H on every qubit: alone (the persistent kernel)
+    and as a group of 64 sets with their own parameters (one launch per pass
+    over all sets), energies and states against the oracle."""
+    print(f"batch every position kind={kind}", flush=True)
+    n = 13
+    ops_k, p0 = circuits.every_position(kind, n, params=True)
+    pre = circuits.OpList()
+    for q in range(n):
+        pre.add(3, q)
+    ops = np.concatenate([pre.array(), ops_k])
+    rng = np.random.default_rng(80 + kind)
+    strings = circuits.random_hamiltonian(rng, n, 10, max_weight=3).pauli_strings
+    circs, params = [], []
+    for c in range(65):
+        ham = circuits.Hamiltonian(strings, rng.normal(0, 1, size=len(strings)))
+        p = p0 + (rng.normal(0, 0.3, size=len(p0)) if c else 0.0)
+        circs.append((n, ops, ham, len(p0)))
+        params.append(p)
+    # circuit 64 differs in structure (an extra gate): it runs in the persistent kernel
+    extra = np.concatenate([ops, circuits.OpList().array()])
+    lone = circuits.OpList()
+    lone.add(0, 12)
+    circs[64] = (n, np.concatenate([extra, lone.array()]), circs[64][2], len(p0))
+    params = np.concatenate(params) if len(p0) else np.zeros(0)
+    b = Batch(circs)
+    assert [g["num_sets"] for g in b.groups()] == [64]
+    out = b.run(params)
+    check_against_oracle(b, circs, params, out, [0, 17, 63, 64])
+    b.close()

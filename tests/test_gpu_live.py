@@ -199,3 +199,25 @@ def test_queued_block_collapse_before_queued_gates():
     assert np.array_equal(q.qubit_map(), o.qubit_map())
     assert q.swap_count() == o.swap_count()
     assert np.array_equal(q.state(), o.physical_state())
+
+
+@pytest.mark.parametrize("n", [12, 13])
+@pytest.mark.parametrize("kind", range(8))
+def test_queued_every_kind_every_position(kind, n):
+    """Queued gates of one kind on each of qubits 0..10 under every control
+    placement: the 12-qubit whole-state launch and the 13-qubit tiled passes
+    (qubits 3..10 take register, lane and wave positions of the tile), bit for
+    bit against the oracle."""
+    print(f"queued every position kind={kind} n={n}", flush=True)
+    ops = circuits.every_position(kind, n)
+    rng = np.random.default_rng(300 + 10 * kind + n)
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    st /= np.linalg.norm(st)
+    q = DenseHip(n, queue=True)
+    q.set_state(st)
+    live(q, ops, n)
+    o = oracle.DenseV2(n)
+    o.set_state(st)
+    o.run(ops)
+    assert np.array_equal(q.state(), o.state())
+    q.close()

@@ -258,12 +258,15 @@ def test_sample_hier_equals_sequential_small():
         assert np.mean(a == b) > 0.999
 
 
-def test_sample_hier_equals_sequential_at_16_and_20_qubits():
+def test_sample_hier_equals_sequential_at_16_20_and_24_qubits():
     """The hierarchical order against the faithful sequential running sum on
-    10^5 draws.  The sequential sum drifts from the exact cumulative weights
-    by 1.8e-11 of the total at 20 qubits, summed over all boundaries (the
-    expected mismatch rate per draw), so these seeded draws all agree."""
-    for n in (16, 20):
+    10^5 draws.  A draw can differ only where x falls between a boundary of
+    one running sum and the same boundary of the other.  Summed over all
+    boundaries, the sequential sum's distance from the exact cumulative
+    weights is 4.9e-11 (16 q), 4.6e-9 (20 q) and 6.1e-7 (24 q) of the total,
+    which bounds the expected mismatch rate per draw.  At 10^5 draws that is
+    at most 0.06 expected mismatches, and these seeded draws all agree."""
+    for n in (16, 20, 24):
         rng = np.random.default_rng(100 + n)
         s = run_oracle(n, circuits.random_circuit(rng, n, 10 * n))
         u = rng.random(100000)

@@ -127,6 +127,12 @@ int32_t kbwh_run(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const doub
  * (default 64: as many as the tile's free positions, 8 for 11-qubit tiles). */
 int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_targets);
 
+/* enable != 0: the handle's adjoint gradients (kbwh_adjoint_gradient,
+ * kbwh_adjoint_batch) run their forward and reverse passes on kernels compiled
+ * for each pass, as KBWH_RUN_COMPILED (no counterpart in the reference).  The
+ * first call of a circuit structure compiles them; values agree with the
+ * generic kernels to rounding. */
+int32_t kbwh_set_compiled(kbwh_state *s, int32_t enable);
 /* Pass-specialised kernels of KBWH_RUN_COMPILED, for the process (no
  * counterpart in the reference): kernels compiled, passes that found theirs
  * already compiled, compile failures, passes run by the generic kernel and
@@ -139,6 +145,10 @@ int32_t kbwh_compiled_stats(uint64_t *kernels_compiled, uint64_t *cache_hits, ui
  * reference).  With code NULL, *size receives the size in bytes. */
 int32_t kbwh_compiled_pass_code(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits, uint32_t pass,
                                 void *code, uint64_t *size);
+/* The same for pass `pass` of the adjoint's reverse sweep of ops (the kernel
+ * kbwh_set_compiled runs), on the host. */
+int32_t kbwh_compiled_reverse_pass_code(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits, uint32_t pass,
+                                        void *code, uint64_t *size);
 
 /* The pass plan of KBWH_RUN_REORDER for a flat state of num_qubits qubits,
  * on the host only (no device needed; no counterpart in the reference):

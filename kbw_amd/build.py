@@ -68,9 +68,10 @@ def build(jobs=8, verbose=False):
         objs.append(obj)
         procs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     # the device tile header as a string, for the kernels compiled at run time
-    emb = embed_header(os.path.join(CSRC, "tile_dev.h"), "kTileDevSource", os.path.join(BUILD_DIR, "tile_dev_src.cpp"))
-    objs.append(emb[:-4] + ".o")
-    procs.append(["g++", *CXX_FLAGS, "-c", emb, "-o", objs[-1]])
+    for hdr, sym in (("tile_dev.h", "kTileDevSource"), ("adj_dev.h", "kAdjDevSource")):
+        emb = embed_header(os.path.join(CSRC, hdr), sym, os.path.join(BUILD_DIR, hdr.replace(".h", "_src.cpp")))
+        objs.append(emb[:-4] + ".o")
+        procs.append(["g++", *CXX_FLAGS, "-c", emb, "-o", objs[-1]])
     running = []
     for cmd in procs:
         running.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
@@ -148,9 +149,10 @@ def build_sanitized(jobs=8, verbose=False):
         obj = os.path.join(SAN_DIR, src.replace(".cpp", ".o"))
         objs.append(obj)
         procs.append([CLANGXX, *cflags, "-c", os.path.join(CSRC, src), "-o", obj])
-    emb = embed_header(os.path.join(CSRC, "tile_dev.h"), "kTileDevSource", os.path.join(SAN_DIR, "tile_dev_src.cpp"))
-    objs.append(emb[:-4] + ".o")
-    procs.append([CLANGXX, *cflags, "-c", emb, "-o", objs[-1]])
+    for hdr, sym in (("tile_dev.h", "kTileDevSource"), ("adj_dev.h", "kAdjDevSource")):
+        emb = embed_header(os.path.join(CSRC, hdr), sym, os.path.join(SAN_DIR, hdr.replace(".h", "_src.cpp")))
+        objs.append(emb[:-4] + ".o")
+        procs.append([CLANGXX, *cflags, "-c", emb, "-o", objs[-1]])
     running = []
     for cmd in procs:
         running.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

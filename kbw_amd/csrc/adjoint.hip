@@ -275,7 +275,8 @@ extern "C" int32_t kbwh_adjoint_gradient(kbwh_state *s, const kbwh_op *ops, uint
     // forward: phi = U |0>, with tiled passes on the phi buffer
     hipError_t e = launch_init(s->adj_phi, size, st);
     if (e != hipSuccess) return hip_status(e, "adjoint init");
-    rc = run_tiled_on(s, s->adj_phi, ops, num_ops, params, true);  // dependency-ordered passes
+    // dependency-ordered passes (on kernels compiled per pass with kbwh_set_compiled)
+    rc = run_tiled_on(s, s->adj_phi, ops, num_ops, params, true, s->compiled_passes);
     if (rc) return rc;
     s->adj_forward = n >= 11 ? s->last_passes : num_ops;
     s->adj_lambda = 0;

@@ -34,6 +34,7 @@
 #define KBWH_TILE_TBITS 8
 #define KBWH_TILE_GATE_RBITS 3
 #include "tile_core.h"
+#include "adj_dev.h"
 
 #pragma clang fp contract(off)
 
@@ -42,76 +43,14 @@ namespace {
 
 using namespace tile;
 
-constexpr uint32_t kMaxSlots = kAdjTiledSlots;  // LDS accumulators: TT doubles each
-constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kMaxSlots = kAdjSlots;
+static_assert(kAdjSlots == kAdjTiledSlots, "adjoint slots per pass");
 constexpr int kTargets = TL - 4;  // free positions for targets (qubit n takes one)
 
 // the parameter index (among the parameterised ops) of each slot of a pass
 struct SlotMap {
     uint32_t global[kMaxSlots];
 };
-
-struct AGate {
-    TGate inv;       // the gate un-applied (the same gate at -angle)
-    uint32_t slot;   // accumulator of the pass, or kNoSlot
-    uint32_t gkind;  // the parameterised gate's kind
-    uint32_t pad[2];
-};
-
-__device__ __forceinline__ double im_conj_mul(dv2 a, dv2 b) { return a.x * b.y - a.y * b.x; }
-
-__device__ __forceinline__ double block_sum(double v, double *ws) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double t = 0.0;
-    for (int w = 0; w < TT / 64; ++w) t += ws[w];
-    __syncthreads();
-    return t;
-}
-
-constexpr int kHalf = TR / 2;  // slots r < kHalf hold phi, r + kHalf lambda
-
-// o[r]: the phi amplitude paired with slot r (r < kHalf) across register bit RB
-template <int RB>
-__device__ __forceinline__ void reg_partners(const dv2 (&v)[TR], dv2 (&o)[kHalf]) {
-#pragma unroll
-    for (int r = 0; r < kHalf; ++r) o[r] = v[r ^ (1 << RB)];
-}
-
-// o[r]: the phi amplitude paired with slot r across tile position tp (a
-// target, so never the lambda bit).  Register bits are dispatched on their
-// bits (no multi-way branch on tp), lane bits go through shuffles and wave
-// bits through LDS, kChunk slots at a time.
-__device__ __forceinline__ void phi_partners(const dv2 (&v)[TR], dv2 (&o)[kHalf], dv2 *lds, uint32_t tid,
-                                             uint32_t tp) {
-    if (tp >= (uint32_t)TB) {
-        const uint32_t rb = tp - TB;
-        if (rb & 2u)
-            reg_partners<2>(v, o);
-        else if (rb & 1u)
-            reg_partners<1>(v, o);
-        else
-            reg_partners<0>(v, o);
-        return;
-    }
-    if (tp < 6) {
-#pragma unroll
-        for (int r = 0; r < kHalf; ++r) o[r] = shfl_xor2(v[r], 1 << tp);
-        return;
-    }
-    const uint32_t pt = tid ^ (1u << tp);
-#pragma unroll
-    for (int q = 0; q < kHalf; q += kChunk) {
-        __syncthreads();  // earlier exchange reads are done
-#pragma unroll
-        for (int r = 0; r < kChunk; ++r) lds[tid + (r << TB)] = v[q + r];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kChunk; ++r) o[q + r] = lds[pt + (r << TB)];
-    }
-}
 
 // v: the tile of the state with lambda's selector qubit, phi in slots
 // r < kHalf and lambda in r + kHalf.  Parameter sets: workgroup w of set
@@ -184,7 +123,7 @@ __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec t
         for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, ts)] = v[r];
     }
     for (uint32_t s = 0; s < nslots; ++s) {
-        const double v = block_sum(acc[s * TT + tid], ws);
+        const double v = adj_block_sum(acc[s * TT + tid], ws);
         if (tid == 0) partial[(size_t)s * gridDim.x + blockIdx.x] = v;
     }
 }
@@ -341,9 +280,18 @@ int32_t plan_reverse(uint32_t n, uint32_t lam_qubit, const kbwh_op *ops, uint64_
     return KBWH_OK;
 }
 
+// the passes' compiled reverse kernels (null where a pass keeps k_adj_tile)
+int32_t compiled_reverse(kbwh_state *s, const std::vector<RPass> &passes, const std::vector<AGate> &table,
+                         std::vector<void *> &fns) {
+    static_assert(sizeof(AGate) == 160, "AGate layout shared with tile_jit.hip");
+    std::vector<JitPass> lst;
+    for (const RPass &P : passes) lst.push_back(JitPass{&table[P.first], P.count, kJitAdjoint});
+    return jit_prepare(s->device, lst, fns);
+}
+
 }  // namespace
 
-// The reverse sweep over phi and lambda (s->adj_phi, lambda right after it)// The reverse sweep over phi and lambda (s->adj_phi, lambda right after it)
+// The reverse sweep over phi and lambda (s->adj_phi, lambda right after it)
 // in tiled passes; slots[j] receives the term of the j-th parameterised op
 // (forward order).  Needs n >= 11.
 int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
@@ -371,11 +319,24 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
     hipError_t e = bytes ? hipMemcpyAsync(s->tile_gates, table.data(), bytes, hipMemcpyHostToDevice, s->stream)
                          : hipSuccess;
     const AGate *dg = (const AGate *)s->tile_gates;
-    for (const RPass &P : passes) {
+    std::vector<void *> fns;
+    if (s->compiled_passes && e == hipSuccess) {
+        if ((prc = compiled_reverse(s, passes, table, fns))) {
+            (void)hipStreamSynchronize(s->stream);
+            return prc;
+        }
+    }
+    for (size_t i = 0; i < passes.size(); ++i) {
+        const RPass &P = passes[i];
         if (e != hipSuccess) break;
-        k_adj_tile<<<grid, TT, 0, s->stream>>>((dv2 *)s->adj_phi, P.ts, dg + P.first, P.count, ntiles, grid,
-                                               P.nslots, partial);
-        e = hipGetLastError();
+        if (!fns.empty() && fns[i]) {
+            e = jit_launch_adj(fns[i], s->adj_phi, &P.ts, sizeof P.ts, dg + P.first, ntiles, grid, partial, grid,
+                               s->stream);
+        } else {
+            k_adj_tile<<<grid, TT, 0, s->stream>>>((dv2 *)s->adj_phi, P.ts, dg + P.first, P.count, ntiles, grid,
+                                                   P.nslots, partial);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess && P.nslots) {
             k_sum_slots<<<dim3(P.nslots, 1), 256, 0, s->stream>>>(partial, grid, grid, 0, slots, P.map);
             e = hipGetLastError();
@@ -435,11 +396,24 @@ int32_t adjoint_reverse_sets(kbwh_state *s, double2 *phi, uint32_t n, uint32_t n
         e = hipGetLastError();
     }
     double *partial = (double *)sb.partial;
-    for (const RPass &P : passes) {
+    std::vector<void *> fns;
+    if (s->compiled_passes && e == hipSuccess) {
+        if ((rc = compiled_reverse(s, passes, table, fns))) {
+            (void)hipStreamSynchronize(st);
+            return rc;
+        }
+    }
+    for (size_t i = 0; i < passes.size(); ++i) {
+        const RPass &P = passes[i];
         if (e != hipSuccess) break;
-        k_adj_tile<<<(uint32_t)grid, TT, 0, st>>>((dv2 *)phi, P.ts, dg + (size_t)nsets * P.first, P.count, tps, wps,
-                                                  P.nslots, partial);
-        e = hipGetLastError();
+        if (!fns.empty() && fns[i]) {
+            e = jit_launch_adj(fns[i], phi, &P.ts, sizeof P.ts, dg + (size_t)nsets * P.first, tps, wps, partial,
+                               (uint32_t)grid, st);
+        } else {
+            k_adj_tile<<<(uint32_t)grid, TT, 0, st>>>((dv2 *)phi, P.ts, dg + (size_t)nsets * P.first, P.count, tps,
+                                                      wps, P.nslots, partial);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess && P.nslots) {
             k_sum_slots<<<dim3(P.nslots, nsets), 256, 0, st>>>(partial, (uint32_t)grid, wps, nparam, slots, P.map);
             e = hipGetLastError();
@@ -453,3 +427,29 @@ int32_t adjoint_reverse_sets(kbwh_state *s, double2 *phi, uint32_t n, uint32_t n
 }
 
 }  // namespace kbwh
+
+// The code object of the compiled reverse-sweep kernel of pass `pass` of a
+// flat state of num_qubits (11..40) qubits, compiled on the host.
+extern "C" int32_t kbwh_compiled_reverse_pass_code(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits,
+                                                   uint32_t pass, void *code, uint64_t *size) {
+    using namespace kbwh;
+    if (!ops || !size) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
+    if (num_qubits < 11 || num_qubits > 40) return set_error(KBWH_ERR_INVALID_ARGUMENT, "11..40 qubits");
+    for (uint64_t i = 0; i < num_ops; ++i)
+        if (ops[i].kind > KRZ || ops[i].target >= num_qubits || (ops[i].ctrl_mask >> num_qubits))
+            return set_error(KBWH_ERR_INVALID_ARGUMENT, "op out of range");
+    int32_t rc;
+    std::vector<RPass> passes;
+    std::vector<AGate> table;
+    std::vector<int32_t> pidx;
+    if ((rc = plan_reverse(num_qubits, num_qubits, ops, num_ops, nullptr, passes, table, pidx))) return rc;
+    if (pass >= passes.size()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "no such pass");
+    std::vector<char> obj;
+    if ((rc = jit_code(JitPass{&table[passes[pass].first], passes[pass].count, kJitAdjoint}, obj))) return rc;
+    if (code) {
+        if (*size < obj.size()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "buffer too small");
+        memcpy(code, obj.data(), obj.size());
+    }
+    *size = obj.size();
+    return KBWH_OK;
+}

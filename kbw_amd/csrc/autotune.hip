@@ -69,6 +69,12 @@ int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_
     return KBWH_OK;
 }
 
+int32_t kbwh_set_compiled(kbwh_state *s, int32_t enable) {
+    if (!s) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null handle");
+    s->compiled_passes = enable != 0;
+    return KBWH_OK;
+}
+
 int32_t kbwh_autotune(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
                       int32_t allow_inexact, kbwh_tune_result *out) {
     if (!s || !out || (num_ops && !ops)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");

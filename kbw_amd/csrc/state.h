@@ -67,6 +67,7 @@ struct kbwh_state {
     // run configuration (kbwh_set_run_config / kbwh_autotune)
     uint32_t fusion_width = 5;   // qubits per fused block
     uint32_t tile_targets = 64;  // most non-diagonal targets above qubit 2 per tiled pass (64: the tile's free positions)
+    bool compiled_passes = false;  // kbwh_set_compiled: the adjoint's passes on kernels compiled per pass
     uint32_t auto_mode = KBWH_RUN_TILED;
     // live gates queued with KBWH_FLAG_QUEUE, applied before the state is read
     bool use_queue = false;
@@ -126,17 +127,24 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
 // Pass-specialised kernels (tile_jit.hip): compiles and loads on `device` the
 // kernel of each pass (cached by the pass's structure and flags); fns[i] is
 // null where pass i runs the generic kernel.
-constexpr uint32_t kJitInit = 1;  // the pass starts from |0...0> of each set instead of a load
+constexpr uint32_t kJitInit = 1;     // the pass starts from |0...0> of each set instead of a load
+constexpr uint32_t kJitAdjoint = 2;  // a reverse-sweep pass of the adjoint (records: the sweep's AGate)
 struct JitPass {
     const void *gates;  // host tile::TGate records of the pass (one set)
     uint32_t count;     // 0: no specialised kernel
     uint32_t flags;
 };
 int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<void *> &fns);
+// the code object of one pass's kernel, compiled on the host (inspection, tests)
+int32_t jit_code(const JitPass &p, std::vector<char> &code);
 // set_shift < 32: per-set records (tile t takes gates + (t >> set_shift) * count);
 // set_mask: the amplitude mask of one set (kJitInit passes)
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
                       hipStream_t stream, uint32_t set_shift = 32, uint64_t set_mask = ~0ull);
+// a compiled reverse-sweep pass (the generic k_adj_tile's arguments; spec: the
+// sweep's 12-qubit tile::TileSpec)
+hipError_t jit_launch_adj(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates, uint32_t tps,
+                          uint32_t wps, double *partial, uint32_t grid, hipStream_t stream);
 void free_adjoint(kbwh_state *s);
 // Tiled reverse sweep of the adjoint gradient (n >= 11): slots[j] receives the
 // term of the j-th parameterised op; partial holds kAdjTiledScratch doubles.

@@ -41,6 +41,7 @@
 namespace kbwh {
 
 extern const char kTileDevSource[];  // tile_dev.h, embedded by the build
+extern const char kAdjDevSource[];   // adj_dev.h, embedded by the build
 
 namespace {
 
@@ -140,21 +141,128 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
     return s;
 }
 
+// The reverse sweep's record (adj_dev.h's AGate; TGate does not depend on
+// the tile width).
+struct AGateRec {
+    TGate inv;
+    uint32_t slot, gkind, pad[2];
+};
+static_assert(sizeof(AGateRec) == 160, "AGate layout");
+constexpr uint32_t kAdjNoSlot = 0xFFFFFFFFu;
+
+std::string adj_pass_key(const AGateRec *g, uint32_t count) {
+    std::string k;
+    const uint32_t fl = kJitAdjoint;
+    k.append((const char *)&fl, sizeof fl);
+    for (uint32_t i = 0; i < count; ++i) {
+        const TGate &G = g[i].inv;
+        const uint64_t f[9] = {G.kind, G.tpos, G.cm, G.ocm, G.otb, G.dlo, G.dhi, g[i].slot, g[i].gkind};
+        k.append((const char *)f, sizeof f);
+    }
+    return k;
+}
+
+// The reverse-sweep kernel of one pass: the generic k_adj_tile's loop over a
+// workgroup's tiles (wps workgroups per set, tps tiles per set, the set's own
+// records gs + set * count), with each gate's term and un-application
+// dispatched at compile time.
+std::string adj_pass_source(const AGateRec *g, uint32_t count, const std::string &name) {
+    uint32_t nslots = 0;
+    for (uint32_t i = 0; i < count; ++i)
+        if (g[i].slot != kAdjNoSlot) nslots = std::max(nslots, g[i].slot + 1);
+    std::string s;
+    s.reserve(2048 + count * 260);
+    s += "#define KBWH_TILE_BITS 12\n#define KBWH_TILE_TBITS 8\n#define KBWH_TILE_GATE_RBITS 3\n"
+         "#include \"adj_dev.h\"\nusing namespace kbwh::tile;\n";
+    s += "extern \"C\" __global__ __launch_bounds__(TT) void " + name +
+         "(dv2 *__restrict__ a, TileSpec ts, const AGate *__restrict__ gs, uint32_t tps, uint32_t wps,\n"
+         "    double *__restrict__ partial) {\n"
+         "  __shared__ dv2 lds[TT * kChunk];\n";
+    const uint32_t ns = std::max<uint32_t>(nslots, 1);
+    s += "  __shared__ double acc[" + std::to_string(ns) + " * TT];\n"
+         "  __shared__ double ws[TT / 64];\n"
+         "  const uint32_t tid = threadIdx.x;\n"
+         "  const uint32_t set = blockIdx.x / wps, wk = blockIdx.x % wps;\n"
+         "  gs += (size_t)set * " + std::to_string(count) + "u;\n";
+    s += "  for (uint32_t s = 0; s < " + std::to_string(ns) + "u; ++s) acc[s * TT + tid] = 0.0;\n"
+         "  const uint64_t tmask = thread_offset(0, tid, ts);\n"
+         "#pragma nounroll\n"
+         "  for (uint32_t tl = wk; tl < tps; tl += wps) {\n"
+         "    const uint32_t t = set * tps + tl;\n"
+         "    const uint64_t base = tile_base(t, ts);\n"
+         "    const uint64_t dt = base | tmask;\n"
+         // the thread index as a new value in every iteration: the per-lane masks
+         // and shuffle addresses the gates derive from it are then computed where
+         // they are used, not hoisted out of the tile loop for every gate at once
+         "    uint32_t tidl = tid;\n"
+         "    asm volatile(\"\" : \"+v\"(tidl));\n"
+         "    dv2 v[TR];\n"
+         "#pragma unroll\n"
+         "    for (int r = 0; r < TR; ++r) v[r] = a[slot_offset(dt, r, ts)];\n";
+    char line[512];
+    for (uint32_t i = 0; i < count; ++i) {
+        const TGate &G = g[i].inv;
+        int m = snprintf(line, sizeof line, "    { const AGate &A = gs[%u]; ", i);
+        if (G.ocm)
+            m += snprintf(line + m, sizeof line - m, "if ((base & %lluull) == %lluull) ", (unsigned long long)G.ocm,
+                          (unsigned long long)G.ocm);
+        m += snprintf(line + m, sizeof line - m, "{ ");
+        if (g[i].slot != kAdjNoSlot)
+            m += snprintf(line + m, sizeof line - m,
+                          "acc[%uu * TT + tid] += adj_term_at<%uu, %uu, %uu>(v, lds, tidl, base, A.inv); ", g[i].slot,
+                          g[i].gkind, G.tpos, G.cm);
+        snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tidl, A.inv); } }\n",
+                 G.kind, G.tpos, G.cm, G.dlo, G.dhi);
+        s += line;
+        // one gate's term and update at a time: scheduled across gates, a
+        // pass of 30-40 gates kept 230-300 registers live and spilled
+        s += "    __builtin_amdgcn_sched_barrier(0);\n";
+    }
+    s += "#pragma unroll\n"
+         "    for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, ts)] = v[r];\n"
+         "  }\n";
+    s += "  for (uint32_t s = 0; s < " + std::to_string(nslots) + "u; ++s) {\n"
+         "    const double x = adj_block_sum(acc[s * TT + tid], ws);\n"
+         "    if (tid == 0) partial[(size_t)s * gridDim.x + blockIdx.x] = x;\n"
+         "  }\n}\n";
+    return s;
+}
+
+// The kernel's fixed scratch (private segment) size from the code object's
+// metadata note (msgpack: the key string, then an unsigned integer); -1 if
+// the key is not found or its value is not an unsigned integer.
+int64_t scratch_bytes(const std::vector<char> &code) {
+    static const char key[] = ".private_segment_fixed_size";
+    const size_t kl = sizeof key - 1;
+    for (size_t i = 0; i + kl + 2 < code.size(); ++i) {
+        if (memcmp(&code[i], key, kl) != 0) continue;
+        const unsigned char *v = (const unsigned char *)&code[i + kl];
+        if (v[0] <= 0x7f) return v[0];
+        if (v[0] == 0xcc && i + kl + 2 <= code.size()) return v[1];
+        if (v[0] == 0xcd && i + kl + 3 <= code.size()) return (int64_t)v[1] << 8 | v[2];
+        if (v[0] == 0xce && i + kl + 5 <= code.size())
+            return (int64_t)v[1] << 24 | (int64_t)v[2] << 16 | (int64_t)v[3] << 8 | v[4];
+        return -1;
+    }
+    return -1;
+}
+
 const char *const kOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
                                 "-Xclang", "-target-feature", "-Xclang",
                                 "-packed-fp32-ops"};
 
 // compiles one pass kernel to a code object; false with a log on failure
-bool compile_source(const std::string &src, std::vector<char> &code, std::string &log) {
+// (also when the kernel uses scratch memory, unless allow_scratch)
+bool compile_source(const std::string &src, std::vector<char> &code, std::string &log, bool allow_scratch = false) {
     const Rtc &r = rtc();
     if (!r.ok) {
         log = r.why;
         return false;
     }
     hiprtcProgram p;
-    const char *hdr[] = {kTileDevSource};
-    const char *inc[] = {"tile_dev.h"};
-    if (r.create(&p, src.c_str(), "kbwh_pass.hip", 1, hdr, inc) != HIPRTC_SUCCESS) {
+    const char *hdr[] = {kTileDevSource, kAdjDevSource};
+    const char *inc[] = {"tile_dev.h", "adj_dev.h"};
+    if (r.create(&p, src.c_str(), "kbwh_pass.hip", 2, hdr, inc) != HIPRTC_SUCCESS) {
         log = "hiprtcCreateProgram failed";
         return false;
     }
@@ -176,8 +284,17 @@ bool compile_source(const std::string &src, std::vector<char> &code, std::string
         ok = r.code(p, code.data()) == HIPRTC_SUCCESS;
     }
     r.destroy(&p);
-    if (!ok) log = "hiprtc returned no code";
-    return ok;
+    if (!ok) {
+        log = "hiprtc returned no code";
+        return false;
+    }
+    // a register tile pushed to scratch memory keeps every result and runs
+    // several times slower: such a kernel is not used
+    if (!allow_scratch && scratch_bytes(code) != 0) {
+        log = "the kernel uses scratch memory";
+        return false;
+    }
+    return true;
 }
 
 constexpr int kMaxDevices = 64;
@@ -222,13 +339,15 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
     std::vector<std::pair<std::shared_ptr<Kernel>, std::string>> todo;
     {
         for (size_t i = 0; i < passes.size(); ++i) {
-            const TGate *g = (const TGate *)passes[i].gates;
             const uint32_t cnt = passes[i].count, fl = passes[i].flags;
             if (cnt == 0 || cnt > kMaxSpecialisedGates) {
                 C.generic++;
                 continue;
             }
-            std::string key = pass_key(g, cnt, fl);
+            const bool adj = (fl & kJitAdjoint) != 0;
+            const TGate *g = (const TGate *)passes[i].gates;
+            const AGateRec *ag = (const AGateRec *)passes[i].gates;
+            std::string key = adj ? adj_pass_key(ag, cnt) : pass_key(g, cnt, fl);
             auto it = C.kernels.find(key);
             if (it != C.kernels.end()) {
                 ks[i] = it->second;
@@ -241,7 +360,7 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
             k->name = nm;
             C.kernels.emplace(key, k);
             ks[i] = k;
-            todo.push_back({k, pass_source(g, cnt, fl, k->name)});
+            todo.push_back({k, adj ? adj_pass_source(ag, cnt, k->name) : pass_source(g, cnt, fl, k->name)});
         }
     }
     if (!todo.empty()) {
@@ -296,6 +415,26 @@ hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gate
     return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
 }
 
+int32_t jit_code(const JitPass &p, std::vector<char> &code) {
+    std::string log;
+    const std::string src = (p.flags & kJitAdjoint) ? adj_pass_source((const AGateRec *)p.gates, p.count, "kbwh_pass")
+                                                    : pass_source((const TGate *)p.gates, p.count, p.flags, "kbwh_pass");
+    if (!compile_source(src, code, log, true))
+        return set_error(KBWH_ERR_EXECUTION_FAILED, "pass kernel did not compile: " + log.substr(0, 400));
+    return KBWH_OK;
+}
+
+hipError_t jit_launch_adj(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates, uint32_t tps,
+                          uint32_t wps, double *partial, uint32_t grid, hipStream_t stream) {
+    alignas(8) unsigned char ts[256];
+    if (spec_bytes > sizeof ts) return hipErrorInvalidValue;
+    memcpy(ts, spec, spec_bytes);
+    dv2 *a = (dv2 *)amp;
+    const void *g = gates;
+    void *args[] = {&a, ts, &g, &tps, &wps, &partial};
+    return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
+}
+
 }  // namespace kbwh
 
 extern "C" int32_t kbwh_compiled_stats(uint64_t *kernels_compiled, uint64_t *cache_hits, uint64_t *failures,
@@ -337,9 +476,8 @@ extern "C" int32_t kbwh_compiled_pass_code(const kbwh_op *ops, uint64_t num_ops,
     std::vector<TGate> g;
     for (const PhysOp &o : cur.ops) g.push_back(make_tgate(o, pos_of, num_qubits));
     std::vector<char> obj;
-    std::string log;
-    if (!compile_source(pass_source(g.data(), (uint32_t)g.size(), 0, "kbwh_pass"), obj, log))
-        return set_error(KBWH_ERR_EXECUTION_FAILED, "pass kernel did not compile: " + log.substr(0, 400));
+    const int32_t rc = jit_code(JitPass{g.data(), (uint32_t)g.size(), 0u}, obj);
+    if (rc) return rc;
     if (code) {
         if (*size < obj.size()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "buffer too small");
         memcpy(code, obj.data(), obj.size());

@@ -631,10 +631,22 @@ int32_t run_sets_forward(kbwh_state *s, double2 *amp, uint32_t n, uint32_t nsets
         e = hipGetLastError();
     }
     const uint32_t ntiles = nsets << (n - TL);
+    std::vector<void *> fns;
+    if (s->compiled_passes && e == hipSuccess) {
+        std::vector<JitPass> lst;
+        for (const SL &L : launches) lst.push_back(JitPass{&tmpl[L.first], L.count, 0u});
+        if ((rc = jit_prepare(s->device, lst, fns))) {
+            (void)hipStreamSynchronize(s->stream);
+            return rc;
+        }
+    }
     for (size_t i = 0; i < launches.size() && e == hipSuccess; ++i) {
         const SL &L = launches[i];
-        e = launch_tile_contracted(amp, &L.ts, d_gates + (size_t)nsets * L.first, L.count, ntiles, s->stream,
-                                   n - TL);
+        if (!fns.empty() && fns[i])
+            e = jit_launch(fns[i], amp, &L.ts, d_gates + (size_t)nsets * L.first, ntiles, s->stream, n - TL);
+        else
+            e = launch_tile_contracted(amp, &L.ts, d_gates + (size_t)nsets * L.first, L.count, ntiles, s->stream,
+                                       n - TL);
     }
     s->last_passes = launches.size();
     s->last_gates = num_ops;

@@ -98,15 +98,16 @@ def compiled_stats():
             "generic_passes": v[3].value, "compile_seconds": sec.value}
 
 
-def compiled_pass_code(ops, num_qubits, pass_index):
-    """The gfx950 code object (bytes) of the RUN_COMPILED kernel of one pass,
-    compiled on the host."""
+def compiled_pass_code(ops, num_qubits, pass_index, reverse=False):
+    """The gfx950 code object (bytes) of the RUN_COMPILED kernel of one pass
+    (reverse=True: of the adjoint's reverse sweep), compiled on the host."""
     a = ops_array(ops)
     size = C.c_uint64()
     lib = _lib.load()
-    check(lib.kbwh_compiled_pass_code(ptr(a), len(a), num_qubits, pass_index, None, C.byref(size)))
+    fn = lib.kbwh_compiled_reverse_pass_code if reverse else lib.kbwh_compiled_pass_code
+    check(fn(ptr(a), len(a), num_qubits, pass_index, None, C.byref(size)))
     buf = C.create_string_buffer(size.value)
-    check(lib.kbwh_compiled_pass_code(ptr(a), len(a), num_qubits, pass_index, buf, C.byref(size)))
+    check(fn(ptr(a), len(a), num_qubits, pass_index, buf, C.byref(size)))
     return buf.raw[:size.value]
 
 
@@ -185,6 +186,10 @@ class DenseHip:
         a = ops_array(ops)
         p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
         check(self._lib.kbwh_run(self._h, ptr(a), len(a), ptr(p), mode))
+
+    def set_compiled(self, enable=True):
+        """Adjoint gradients on kernels compiled per pass (kbwh_set_compiled)."""
+        check(self._lib.kbwh_set_compiled(self._h, 1 if enable else 0))
 
     def set_run_config(self, fusion_width=5, tile_targets=64):
         """Qubits per fused block (4 or 5) and most non-diagonal targets per

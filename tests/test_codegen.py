@@ -109,3 +109,21 @@ def test_compiled_pass_kernels_keep_registers(tmp_path, pass_index):
     meta = _notes(co)
     assert meta["private_segment_fixed_size"] == 0, meta
     assert meta["vgpr_count"] is not None and meta["vgpr_count"] <= 96, meta
+
+
+@pytest.mark.parametrize("pass_index", [0, 3, 12, 24])
+def test_compiled_reverse_pass_kernels_use_no_scratch(tmp_path, pass_index):
+    """Reverse-sweep kernels compiled for passes of the 24-qubit, 576-parameter
+    ansatz gradient: no scratch memory (a kernel that spills is not used at run
+    time, and its pass falls back to the generic kernel)."""
+    if not os.path.exists(LIB):
+        pytest.skip("kbw_amd/libkbwh.so is not built")
+    if not _tools():
+        pytest.skip("llvm-readelf not available")
+    from kbw_amd import circuits, compiled_pass_code
+
+    ops, _ = circuits.hea(24, 12, seed=7)
+    co = tmp_path / "rev.co"
+    co.write_bytes(compiled_pass_code(ops, 24, pass_index, reverse=True))
+    meta = _notes(co)
+    assert meta["private_segment_fixed_size"] == 0, meta

@@ -127,13 +127,14 @@ def test_torch_autograd_function():
     assert float(e) == pytest.approx(oracle_energy(n, ops, params, ham), abs=1e-12)
 
 
+@pytest.mark.parametrize("compiled", [False, True])
 @pytest.mark.parametrize("kind", [4, 5, 6, 7])
-def test_adjoint_every_position_and_control(kind):
+def test_adjoint_every_position_and_control(kind, compiled):
     """Parameterised P / RX / RY / RZ on each of qubits 0..10 of a 13-qubit
     state (every tile position of the reverse sweep) under every control
     placement, after H on every qubit and with X, Y, H between: gradients
     against central differences of the oracle's energy (step 1e-6)."""
-    print(f"adjoint every position kind={kind}", flush=True)
+    print(f"adjoint every position kind={kind} compiled={compiled}", flush=True)
     n = 13
     ops_k, params = circuits.every_position(kind, n, params=True)
     pre = circuits.OpList()
@@ -146,6 +147,7 @@ def test_adjoint_every_position_and_control(kind):
     rng = np.random.default_rng(60 + kind)
     ham = circuits.random_hamiltonian(rng, n, 12, max_weight=3)
     sim = DenseHip(n)
+    sim.set_compiled(compiled)
     e, g = sim.adjoint_gradient(ops, params, ham)
     assert e == pytest.approx(oracle_energy(n, ops, params, ham), abs=1e-12)
     h = 1e-6
@@ -156,4 +158,29 @@ def test_adjoint_every_position_and_control(kind):
         p[j] -= 2 * h
         fm = oracle_energy(n, ops, p, ham)
         assert abs(g[j] - (fp - fm) / (2 * h)) <= 2e-8, (j, g[j], (fp - fm) / (2 * h))
+    sim.close()
+
+
+@pytest.mark.parametrize("n", [13, 20])
+def test_adjoint_compiled_equals_generic(n):
+    """kbwh_set_compiled: the forward and reverse passes on compiled kernels
+    give the generic kernels' energy and gradient to rounding, and a second
+    call with other parameters compiles nothing."""
+    from kbw_amd import compiled_stats
+
+    print(f"adjoint compiled n={n}", flush=True)
+    ops, params = circuits.hea(n, 4, seed=50 + n)
+    ham = circuits.xxz(n, seed=51 + n)
+    sim = DenseHip(n)
+    e0, g0 = sim.adjoint_gradient(ops, params, ham)
+    sim.set_compiled(True)
+    e1, g1 = sim.adjoint_gradient(ops, params, ham)
+    assert abs(e1 - e0) <= 1e-12 and np.max(np.abs(g1 - g0)) <= 1e-12
+    before = compiled_stats()
+    p2 = params + 0.1
+    e2, g2 = sim.adjoint_gradient(ops, p2, ham)
+    assert compiled_stats()["compiled"] == before["compiled"]
+    sim.set_compiled(False)
+    e3, g3 = sim.adjoint_gradient(ops, p2, ham)
+    assert abs(e2 - e3) <= 1e-12 and np.max(np.abs(g2 - g3)) <= 1e-12
     sim.close()

@@ -654,7 +654,7 @@ def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12, compiled=Fa
     if compiled:
         out["roofline"].update({"traffic": TRAFFIC_COMPILED,
                                 "kernel": "kbwh_pass_* (a kernel compiled per dependency-ordered pass, "
-                                          "11-qubit register tiles)"})
+                                          "12-qubit register tiles, searched plan)"})
         out["compile"] = {"kernels": after["compiled"] - before["compiled"],
                           "failures": after["failures"] - before["failures"],
                           "generic_passes": after["generic_passes"] - before["generic_passes"],

@@ -79,8 +79,11 @@ extern "C" {
                                 tile positions and controls fixed in its code; angles read from the gate records):
                                 the first run of a pass structure compiles its kernels at run time (hiprtc, in
                                 parallel host threads) and later runs reuse them for the process; amplitudes equal to
-                                KBWH_RUN_REORDER's to rounding.  A pass that does not compile, or of more than 256
-                                gates, runs the generic kernel; a Block state runs as KBWH_RUN_REORDER */
+                                KBWH_RUN_REORDER's to rounding.  From 12 qubits on the passes use 12-qubit tiles
+                                (fewer passes) and the plan search (kbwh_set_plan_search); if one of those kernels
+                                does not compile the run takes 11-qubit passes, where a pass that does not compile,
+                                or of more than 256 gates, runs the generic kernel.  A Block state runs as
+                                KBWH_RUN_REORDER */
 
 /* One circuit operation (24 bytes).  `angle` is used unless `param` >= 0
  * and a parameter vector is supplied, in which case the angle is
@@ -133,6 +136,14 @@ int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_
  * first call of a circuit structure compiles them; values agree with the
  * generic kernels to rounding. */
 int32_t kbwh_set_compiled(kbwh_state *s, int32_t enable);
+/* Plan search of the dependency-ordered runs (KBWH_RUN_REORDER,
+ * KBWH_RUN_COMPILED; no counterpart in the reference): besides the greedy
+ * plan, `trials` randomised plans are tried and the one with the fewest passes
+ * is kept, once per circuit structure (plans are cached for the process; the
+ * angles do not change a plan).  trials < 0 restores the default: 0 for
+ * KBWH_RUN_REORDER, 256 for KBWH_RUN_COMPILED (about 0.1 s of host time on the
+ * first call of a 900-gate, 30-qubit circuit).  At most 65536. */
+int32_t kbwh_set_plan_search(kbwh_state *s, int32_t trials);
 /* Pass-specialised kernels of KBWH_RUN_COMPILED, for the process (no
  * counterpart in the reference): kernels compiled, passes that found theirs
  * already compiled, compile failures, passes run by the generic kernel and
@@ -141,8 +152,9 @@ int32_t kbwh_compiled_stats(uint64_t *kernels_compiled, uint64_t *cache_hits, ui
                             uint64_t *generic_passes, double *compile_seconds);
 /* The gfx950 code object of the specialised kernel of pass `pass` of the
  * KBWH_RUN_COMPILED plan of ops on a flat state of num_qubits (11..40)
- * qubits, compiled on the host (no device needed; no counterpart in the
- * reference).  With code NULL, *size receives the size in bytes. */
+ * qubits (12-qubit tiles from 12 qubits on, with the default plan search),
+ * compiled on the host (no device needed; no counterpart in the reference).
+ * With code NULL, *size receives the size in bytes. */
 int32_t kbwh_compiled_pass_code(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits, uint32_t pass,
                                 void *code, uint64_t *size);
 /* The same for pass `pass` of the adjoint's reverse sweep of ops (the kernel

@@ -80,6 +80,7 @@ SIGNATURES = {
     "kbwh_batch_plan": [_p, _u32, _p, _p, _p, _p, _p, _u64, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)],
     "kbwh_reorder_plan": [_p, _u64, _u32, _u32, _p, _p, C.POINTER(_u64)],
     "kbwh_set_compiled": [_vp, _i32],
+    "kbwh_set_plan_search": [_vp, _i32],
     "kbwh_compiled_stats": [C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_f64)],
     "kbwh_compiled_pass_code": [_p, _u64, _u32, _u32, _p, C.POINTER(_u64)],
     "kbwh_compiled_reverse_pass_code": [_p, _u64, _u32, _u32, _p, C.POINTER(_u64)],

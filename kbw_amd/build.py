@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libkbwh.so")
 BUILD_DIR = os.path.join(ROOT, "_build", "kbwh")
 
-HIP_SOURCES = ["kernels.hip", "state.hip", "observables.hip", "tiled.hip", "adjoint.hip", "batch.hip", "fused.hip", "autotune.hip", "live.hip", "adjoint_tiled.hip", "tiled_fma.hip", "tiled_pass.hip", "tile_jit.hip"]
+HIP_SOURCES = ["kernels.hip", "state.hip", "observables.hip", "tiled.hip", "adjoint.hip", "batch.hip", "fused.hip", "autotune.hip", "live.hip", "adjoint_tiled.hip", "tiled_fma.hip", "tiled_pass.hip", "tile_jit.hip", "tiled_wide.hip"]
 CXX_SOURCES = ["coef.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

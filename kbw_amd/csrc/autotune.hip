@@ -75,6 +75,13 @@ int32_t kbwh_set_compiled(kbwh_state *s, int32_t enable) {
     return KBWH_OK;
 }
 
+int32_t kbwh_set_plan_search(kbwh_state *s, int32_t trials) {
+    if (!s) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null handle");
+    if (trials > 65536) return set_error(KBWH_ERR_INVALID_ARGUMENT, "at most 65536 trials");
+    s->plan_trials = trials < 0 ? -1 : trials;
+    return KBWH_OK;
+}
+
 int32_t kbwh_autotune(kbwh_state *s, const kbwh_op *ops, uint64_t num_ops, const double *params,
                       int32_t allow_inexact, kbwh_tune_result *out) {
     if (!s || !out || (num_ops && !ops)) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");

@@ -68,6 +68,7 @@ struct kbwh_state {
     uint32_t fusion_width = 5;   // qubits per fused block
     uint32_t tile_targets = 64;  // most non-diagonal targets above qubit 2 per tiled pass (64: the tile's free positions)
     bool compiled_passes = false;  // kbwh_set_compiled: the adjoint's passes on kernels compiled per pass
+    int32_t plan_trials = -1;       // kbwh_set_plan_search: randomised plans tried (-1: 0 reorder, 256 compiled)
     uint32_t auto_mode = KBWH_RUN_TILED;
     // live gates queued with KBWH_FLAG_QUEUE, applied before the state is read
     bool use_queue = false;
@@ -129,6 +130,7 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
 // null where pass i runs the generic kernel.
 constexpr uint32_t kJitInit = 1;     // the pass starts from |0...0> of each set instead of a load
 constexpr uint32_t kJitAdjoint = 2;  // a reverse-sweep pass of the adjoint (records: the sweep's AGate)
+constexpr uint32_t kJitWide = 4;     // 12-qubit tiles (records and TileSpec of the wide planner)
 struct JitPass {
     const void *gates;  // host tile::TGate records of the pass (one set)
     uint32_t count;     // 0: no specialised kernel
@@ -141,6 +143,20 @@ int32_t jit_code(const JitPass &p, std::vector<char> &code);
 // set_mask: the amplitude mask of one set (kJitInit passes)
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
                       hipStream_t stream, uint32_t set_shift = 32, uint64_t set_mask = ~0ull);
+// a wide pass (spec: the 12-qubit tile::TileSpec of spec_bytes bytes)
+hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates,
+                           uint32_t ntiles, hipStream_t stream);
+// Dependency-ordered runs in 12-qubit tiles with pass-specialised kernels
+// (tiled_wide.hip, KBWH_RUN_COMPILED on flat states of 12 qubits and more).
+// *done is false when the run was left to the 11-qubit passes (a pass kernel
+// did not compile; nothing was launched).
+constexpr uint32_t kWideTileQubits = 12;
+int32_t run_compiled_wide(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops, const double *params,
+                          bool *done);
+// the gate records (tile::TGate of the wide unit) of pass `pass` of the wide
+// plan of ops on num_qubits qubits (host only; code inspection)
+int32_t wide_pass_records(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits, uint32_t pass,
+                          std::vector<unsigned char> &recs, uint32_t &count);
 // a compiled reverse-sweep pass (the generic k_adj_tile's arguments; spec: the
 // sweep's 12-qubit tile::TileSpec)
 hipError_t jit_launch_adj(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates, uint32_t tps,

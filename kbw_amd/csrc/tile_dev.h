@@ -558,10 +558,71 @@ __device__ __forceinline__ uint64_t slot_offset(uint64_t dt, int r, const TileSp
 // pass instantiates only the path each gate takes.  The coefficients still
 // come from the record.
 
-template <uint32_t TP, int PK, bool CTRL>
+// Exchange of lane bit LB (4 or 5) with register bit K.  v_permlane16_swap
+// (LB 4) swaps the odd 16-lane rows of its first register with the even rows
+// of its second, v_permlane32_swap (LB 5) the upper 32 lanes of the first with
+// the lower 32 of the second.  Applied to the register pair (r, r | 1 << K)
+// of every slot r with bit K clear, the amplitude at lane bit LB = x and
+// register bit K = y moves to lane bit LB = y and register bit K = x: lane bit
+// LB and register bit K trade places, every other bit stays.  Applying it
+// twice restores the layout.  Four swaps per register pair, no LDS traffic
+// (the lane exchange through ds_bpermute costs four LDS permutes per
+// amplitude and gate).
+template <int LB>
+__device__ __forceinline__ void swap_rows(double &p, double &q) {
+    const unsigned long long pb = (unsigned long long)__double_as_longlong(p);
+    const unsigned long long qb = (unsigned long long)__double_as_longlong(q);
+    unsigned plo = (unsigned)pb, phi = (unsigned)(pb >> 32), qlo = (unsigned)qb, qhi = (unsigned)(qb >> 32);
+    if constexpr (LB == 4) {
+        const auto lo = __builtin_amdgcn_permlane16_swap(plo, qlo, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap(phi, qhi, false, false);
+        plo = lo[0]; qlo = lo[1]; phi = hi[0]; qhi = hi[1];
+    } else {
+        const auto lo = __builtin_amdgcn_permlane32_swap(plo, qlo, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(phi, qhi, false, false);
+        plo = lo[0]; qlo = lo[1]; phi = hi[0]; qhi = hi[1];
+    }
+    p = __longlong_as_double((long long)(((unsigned long long)phi << 32) | plo));
+    q = __longlong_as_double((long long)(((unsigned long long)qhi << 32) | qlo));
+}
+
+template <int LB, int K>
+__device__ __forceinline__ void swap_lane_reg(dv2 (&v)[TR]) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        if ((r >> K) & 1) continue;
+        const int r1 = r | (1 << K);
+        double ax = v[r].x, ay = v[r].y, bx = v[r1].x, by = v[r1].y;
+        swap_rows<LB>(ax, bx);
+        swap_rows<LB>(ay, by);
+        v[r] = mk2(ax, ay);
+        v[r1] = mk2(bx, by);
+    }
+}
+
+// A gate register bit that is not among the in-tile controls CM (-1: none)
+constexpr int free_gate_reg_bit(uint32_t cm) {
+    for (int k = TGR - 1; k >= 0; --k)
+        if (!((cm >> (TB + k)) & 1u)) return k;
+    return -1;
+}
+
+// Target on lane bit 4 or 5 (the ds_bpermute lanes): the target trades places
+// with a free register bit K, the gate runs as a register-bit gate (both
+// amplitudes of each pair in one thread, the half known per slot), and the
+// layout is restored.  Controls keep their positions (K is not one of them,
+// the target never is), so the control tests are unchanged; the outputs are
+// the register-bit form's values, equal to the lane form's (the lane form's
+// sign flips give the same operands).
+template <uint32_t TP, int PK, bool CTRL, uint32_t CM>
 __device__ __forceinline__ void pair_at(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
     if constexpr (TP >= (uint32_t)TB) {
         reg_pair<(int)(TP - TB), PK, CTRL>(v, tid, G);
+    } else if constexpr ((TP == 4 || TP == 5) && free_gate_reg_bit(CM) >= 0) {
+        constexpr int K = free_gate_reg_bit(CM);
+        swap_lane_reg<(int)TP, K>(v);
+        reg_pair<K, PK, CTRL>(v, tid, G);
+        swap_lane_reg<(int)TP, K>(v);
     } else if constexpr (TP < 6) {
         constexpr int LX = (TP & 4u) ? LX_SHFL : (TP & 2u) ? ((TP & 1u) ? LX_R8 : LX_R4) : ((TP & 1u) ? LX_Q2 : LX_Q1);
         lane_pair<PK, CTRL, LX>(v, tid, TP, G);
@@ -629,17 +690,17 @@ __device__ __forceinline__ void gate_at(dv2 (&v)[TR], dv2 *lds, uint64_t base, u
     } else if constexpr (KIND == KRZ) {
         diag_at<DK_RZ, TP, CM, DLO, DHI>(v, base, tid, G);
     } else if constexpr (KIND == KX) {
-        pair_at<TP, PK_X, CM != 0>(v, lds, tid, G);
+        pair_at<TP, PK_X, CM != 0, CM>(v, lds, tid, G);
     } else if constexpr (CM != 0) {
-        pair_at<TP, PK_GEN, true>(v, lds, tid, G);
+        pair_at<TP, PK_GEN, true, CM>(v, lds, tid, G);
     } else if constexpr (KIND == KY) {
-        pair_at<TP, PK_Y, false>(v, lds, tid, G);
+        pair_at<TP, PK_Y, false, CM>(v, lds, tid, G);
     } else if constexpr (KIND == KH) {
-        pair_at<TP, PK_H, false>(v, lds, tid, G);
+        pair_at<TP, PK_H, false, CM>(v, lds, tid, G);
     } else if constexpr (KIND == KRX) {
-        pair_at<TP, PK_RX, false>(v, lds, tid, G);
+        pair_at<TP, PK_RX, false, CM>(v, lds, tid, G);
     } else {
-        pair_at<TP, PK_TWO, false>(v, lds, tid, G);
+        pair_at<TP, PK_TWO, false, CM>(v, lds, tid, G);
     }
 }
 

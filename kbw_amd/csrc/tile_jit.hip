@@ -108,9 +108,14 @@ uint64_t fnv1a(const std::string &s) {
 // the records gp + (t >> set_shift) * count) and, for an initialising pass
 // (JIT_INIT), the amplitude mask of one set: the tile starts as |0...0> of
 // its set in registers instead of being loaded.
-std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const std::string &name) {
+// A wide pass (JIT_WIDE) runs 12-qubit tiles, 16 amplitudes per thread,
+// built for `waves` waves per SIMD (the register budget: 3 gives 168 VGPRs).
+std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const std::string &name, int waves = 3) {
     std::string s;
     s.reserve(1024 + count * 200);
+    if (flags & kJitWide)
+        s += "#define KBWH_TILE_BITS 12\n#define KBWH_TILE_TBITS 8\n#define KBWH_TILE_WAVES " + std::to_string(waves) +
+             "\n";
     s += "#define KBWH_TILE_CONTRACT 1\n#include \"tile_dev.h\"\nusing namespace kbwh::tile;\n";
     s += "extern \"C\" __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, "
          "KBWH_TILE_WAVES))) void ";
@@ -118,7 +123,10 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
     s += "(dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gp, uint32_t ntiles, uint32_t set_shift,\n"
          "    uint64_t set_mask) {\n"
          "  __shared__ dv2 lds[TT * kChunk];\n"
-         "  const uint32_t tid = threadIdx.x, tile = blockIdx.x;\n"
+         "  const uint32_t tid = threadIdx.x;\n"
+         // workgroups are dispatched round-robin over the 8 XCDs: those of one
+         // XCD take consecutive tiles, whose amplitudes share DRAM rows
+         "  const uint32_t tile = (ntiles & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (ntiles >> 3) + (blockIdx.x >> 3);\n"
          "  if (tile >= ntiles) return;\n";
     s += "  if (set_shift < 32) gp += (size_t)(tile >> set_shift) * " + std::to_string(count) + "u;\n";
     s += "  const uint64_t base = tile_base(tile, ts);\n"
@@ -297,6 +305,19 @@ bool compile_source(const std::string &src, std::vector<char> &code, std::string
     return true;
 }
 
+// The kernel of one pass under `name`.  A wide pass that spills at 3 waves
+// per SIMD is built again for 2 (256 VGPRs); allow_scratch keeps the last
+// attempt's code even if it spills (inspection).
+bool compile_pass(const JitPass &p, const std::string &name, std::vector<char> &code, std::string &log,
+                  bool allow_scratch) {
+    if (p.flags & kJitAdjoint)
+        return compile_source(adj_pass_source((const AGateRec *)p.gates, p.count, name), code, log, allow_scratch);
+    const TGate *g = (const TGate *)p.gates;
+    if (!(p.flags & kJitWide)) return compile_source(pass_source(g, p.count, p.flags, name), code, log, allow_scratch);
+    if (compile_source(pass_source(g, p.count, p.flags, name, 3), code, log, false)) return true;
+    return compile_source(pass_source(g, p.count, p.flags, name, 2), code, log, allow_scratch);
+}
+
 constexpr int kMaxDevices = 64;
 
 struct Kernel {
@@ -336,7 +357,11 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
     // one prepare at a time: a kernel in the cache is compiled (or failed)
     std::lock_guard<std::mutex> lk(C.mu);
     std::vector<std::shared_ptr<Kernel>> ks(passes.size());
-    std::vector<std::pair<std::shared_ptr<Kernel>, std::string>> todo;
+    struct Todo {
+        std::shared_ptr<Kernel> k;
+        JitPass p;
+    };
+    std::vector<Todo> todo;
     {
         for (size_t i = 0; i < passes.size(); ++i) {
             const uint32_t cnt = passes[i].count, fl = passes[i].flags;
@@ -360,7 +385,7 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
             k->name = nm;
             C.kernels.emplace(key, k);
             ks[i] = k;
-            todo.push_back({k, adj ? adj_pass_source(ag, cnt, k->name) : pass_source(g, cnt, fl, k->name)});
+            todo.push_back({k, passes[i]});
         }
     }
     if (!todo.empty()) {
@@ -368,8 +393,8 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
         std::atomic<size_t> next{0};
         auto work = [&] {
             for (size_t j; (j = next.fetch_add(1)) < todo.size();) {
-                Kernel &k = *todo[j].first;
-                k.failed = !compile_source(todo[j].second, k.code, k.log);
+                Kernel &k = *todo[j].k;
+                k.failed = !compile_pass(todo[j].p, k.name, k.code, k.log, false);
             }
         };
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -381,7 +406,7 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
         C.compile_us += (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
                             std::chrono::steady_clock::now() - t0)
                             .count();
-        for (auto &w : todo) (w.first->failed ? C.failures : C.compiled)++;
+        for (auto &w : todo) (w.k->failed ? C.failures : C.compiled)++;
     }
     // modules load into the current device: make the state's device current
     int cur = -1;
@@ -415,11 +440,22 @@ hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gate
     return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
 }
 
+hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates,
+                           uint32_t ntiles, hipStream_t stream) {
+    alignas(8) unsigned char ts[256];
+    if (spec_bytes > sizeof ts) return hipErrorInvalidValue;
+    memcpy(ts, spec, spec_bytes);
+    dv2 *a = (dv2 *)amp;
+    const void *g = gates;
+    uint32_t set_shift = 32;
+    uint64_t set_mask = ~0ull;
+    void *args[] = {&a, ts, &g, &ntiles, &set_shift, &set_mask};
+    return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
+}
+
 int32_t jit_code(const JitPass &p, std::vector<char> &code) {
     std::string log;
-    const std::string src = (p.flags & kJitAdjoint) ? adj_pass_source((const AGateRec *)p.gates, p.count, "kbwh_pass")
-                                                    : pass_source((const TGate *)p.gates, p.count, p.flags, "kbwh_pass");
-    if (!compile_source(src, code, log, true))
+    if (!compile_pass(p, "kbwh_pass", code, log, true))
         return set_error(KBWH_ERR_EXECUTION_FAILED, "pass kernel did not compile: " + log.substr(0, 400));
     return KBWH_OK;
 }
@@ -457,6 +493,21 @@ extern "C" int32_t kbwh_compiled_pass_code(const kbwh_op *ops, uint64_t num_ops,
     using namespace kbwh;
     if (!ops || !size) return set_error(KBWH_ERR_INVALID_ARGUMENT, "null argument");
     if (num_qubits < TL || num_qubits > 40) return set_error(KBWH_ERR_INVALID_ARGUMENT, "11..40 qubits");
+    if (num_qubits >= kWideTileQubits) {
+        // the wide tiles of KBWH_RUN_COMPILED
+        std::vector<unsigned char> recs;
+        uint32_t count = 0;
+        int32_t rc = wide_pass_records(ops, num_ops, num_qubits, pass, recs, count);
+        if (rc) return rc;
+        std::vector<char> obj;
+        if ((rc = jit_code(JitPass{recs.data(), count, kJitWide}, obj))) return rc;
+        if (code) {
+            if (*size < obj.size()) return set_error(KBWH_ERR_INVALID_ARGUMENT, "buffer too small");
+            memcpy(code, obj.data(), obj.size());
+        }
+        *size = obj.size();
+        return KBWH_OK;
+    }
     std::vector<PhysOp> phys(num_ops);
     for (uint64_t i = 0; i < num_ops; ++i) {
         const kbwh_op &op = ops[i];

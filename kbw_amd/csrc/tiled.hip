@@ -691,6 +691,11 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
     s->last_passes = 0;
     int32_t rc = check_ops(n, ops, num_ops);
     if (rc) return rc;
+    if (compiled && reorder && !s->block && n >= kWideTileQubits) {
+        bool done = false;
+        rc = run_compiled_wide(s, amp, ops, num_ops, params, &done);
+        if (rc || done) return rc;
+    }
     PassBuilder pb(s, amp, reorder && !s->block);
     pb.compiled = compiled && reorder && !s->block;
     // the map replayed in program order, exactly as gate by gate: a Block
@@ -755,7 +760,10 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         }
         fuse_diagonals(phys, n);
         std::vector<uint32_t> order, ends;
-        schedule_passes(phys, n, pb.cap, order, ends);
+        // the plan search: by default for compiled runs only (their first
+        // call compiles kernels anyway); plans are kept per structure
+        const uint32_t trials = s->plan_trials >= 0 ? (uint32_t)s->plan_trials : (pb.compiled ? 256u : 0u);
+        schedule_passes_cached(phys, n, pb.cap, trials, order, ends);
         size_t k = 0;
         for (const uint32_t end : ends) {
             for (; k < end; ++k) add_op(pb.cur, phys[order[k]]);

@@ -191,6 +191,12 @@ class DenseHip:
         """Adjoint gradients on kernels compiled per pass (kbwh_set_compiled)."""
         check(self._lib.kbwh_set_compiled(self._h, 1 if enable else 0))
 
+    def set_plan_search(self, trials):
+        """Randomised plans tried by dependency-ordered runs, once per circuit
+        structure (kbwh_set_plan_search; < 0: the default, 0 for RUN_REORDER
+        and 256 for RUN_COMPILED)."""
+        check(self._lib.kbwh_set_plan_search(self._h, int(trials)))
+
     def set_run_config(self, fusion_width=5, tile_targets=64):
         """Qubits per fused block (4 or 5) and most non-diagonal targets per
         tiled pass (1..64; 64 = as many as the tile has free positions)."""

@@ -92,11 +92,12 @@ def _notes(path):
     return out
 
 
-@pytest.mark.parametrize("pass_index", [0, 5, 16])
+@pytest.mark.parametrize("pass_index", [0, 7, 15])
 def test_compiled_pass_kernels_keep_registers(tmp_path, pass_index):
     """Kernels compiled at run time for single passes of the 30-qubit brickwork
-    (50, 92 and 67 gates): no scratch memory, and within the pass kernels'
-    register budget."""
+    (12-qubit tiles, 16 planned passes): no scratch memory, and within the
+    register budget of 2 waves per SIMD (256 VGPRs; a kernel that spills at 3
+    waves is built again for 2)."""
     if not os.path.exists(LIB):
         pytest.skip("kbw_amd/libkbwh.so is not built")
     if not _tools():
@@ -106,6 +107,23 @@ def test_compiled_pass_kernels_keep_registers(tmp_path, pass_index):
     ops, _ = circuits.brickwork(30, 20, seed=1)
     co = tmp_path / "pass.co"
     co.write_bytes(compiled_pass_code(ops, 30, pass_index))
+    meta = _notes(co)
+    assert meta["private_segment_fixed_size"] == 0, meta
+    assert meta["vgpr_count"] is not None and meta["vgpr_count"] <= 256, meta
+
+
+def test_compiled_narrow_pass_kernels_keep_registers(tmp_path):
+    """On 11 qubits (one 11-qubit tile) the compiled passes keep the 11-qubit
+    kernels' budget: no scratch memory, at most 96 VGPRs."""
+    if not os.path.exists(LIB):
+        pytest.skip("kbw_amd/libkbwh.so is not built")
+    if not _tools():
+        pytest.skip("llvm-readelf not available")
+    from kbw_amd import circuits, compiled_pass_code
+
+    ops, _ = circuits.brickwork(11, 6, seed=1)
+    co = tmp_path / "pass.co"
+    co.write_bytes(compiled_pass_code(ops, 11, 0))
     meta = _notes(co)
     assert meta["private_segment_fixed_size"] == 0, meta
     assert meta["vgpr_count"] is not None and meta["vgpr_count"] <= 96, meta

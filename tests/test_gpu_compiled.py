@@ -4,8 +4,11 @@ the oracle and the dependency-ordered generic kernel (KBWH_RUN_REORDER).
 A compiled pass runs the same gate arithmetic as the contracted generic pass
 kernel, with each gate's kind, tile position and masks fixed in its code, so
 it agrees with the per-gate path to rounding (1e-13 on a unit-norm state) and
-with KBWH_RUN_REORDER to rounding.  Each case puts every gate of one kind into
-one pass, so a kind's every tile position and control placement compiles once.
+with KBWH_RUN_REORDER to rounding.  From 12 qubits on the compiled passes use
+12-qubit tiles (16 amplitudes per thread) and a searched plan, so they may
+take fewer passes than KBWH_RUN_REORDER; on 11 qubits they use the 11-qubit
+tile.  Each case puts every gate of one kind into one pass, so a kind's every
+tile position and control placement compiles once.
 """
 import math
 
@@ -35,21 +38,23 @@ def run_modes(n, st, ops, modes):
 
 
 @pytest.mark.parametrize("kind", range(8))
-def test_compiled_every_position_and_control_placement(kind):
-    """One pass of 44 gates of `kind` on a 13-qubit state: targets on all 11
-    tile qubits (lane bits 0..5, wave bits 6..7, register bits 8..10 in some
-    bijection), each uncontrolled, with a control on a lane qubit, on a tile
-    qubit 3..10 and outside the tile (qubit 12); angles covering every sign
-    and magnitude form of the half-angle coefficients."""
-    print(f"compiled kind {kind}", flush=True)
-    n = 13
-    rng = np.random.default_rng(140 + kind)
+@pytest.mark.parametrize("n,tile", [(13, 12), (11, 11)])
+def test_compiled_every_position_and_control_placement(kind, n, tile):
+    """One pass of gates of `kind` whose targets cover every tile qubit: on 13
+    qubits the 12-qubit tile (lane bits 0..5, wave bits 6..7, register bits
+    8..11 in some bijection), on 11 qubits the 11-qubit tile (register bits
+    8..10); each target uncontrolled, with a control on a lane qubit, on a
+    tile qubit 3.. and, on 13 qubits, outside the tile (qubit 12); angles
+    covering every sign and magnitude form of the half-angle coefficients."""
+    print(f"compiled kind {kind} n={n}", flush=True)
+    rng = np.random.default_rng(140 + kind + n)
     st = random_state(rng, n)
     ops = circuits.OpList()
-    for q in range(11):
+    for q in range(tile):
         lane_c = [0] if q != 0 else [1]
         tile_c = [3] if q != 3 else [4]
-        for ctrl, ang in zip(([], lane_c, tile_c, [12]), (0.9, 2.5, 4.0, 5.9)):
+        outside = [n - 1] if n > tile else [5 if q != 5 else 6]
+        for ctrl, ang in zip(([], lane_c, tile_c, outside), (0.9, 2.5, 4.0, 5.9)):
             ops.add(kind, q, ctrl, angle=ang + 0.01 * q)
     arr = ops.array()
     (ref, _), (comp, passes) = run_modes(n, st, arr, (RUN_PER_GATE, RUN_COMPILED))
@@ -84,7 +89,7 @@ def test_compiled_diagonal_masks_in_one_pass():
 def test_compiled_brickwork_against_oracle_and_reorder(n, layers, seed):
     """The brickwork circuit (every gate kind, CZ/CNOT, multi-controlled X)
     from |0>: amplitudes against the oracle's program-order run to 1e-12, and
-    against KBWH_RUN_REORDER's generic passes to 1e-13, with the same passes."""
+    against KBWH_RUN_REORDER's generic passes to 1e-13, in no more passes."""
     print(f"compiled brickwork n={n}", flush=True)
     ops = circuits.OpList()
     circuits.brickwork_layers(ops, np.random.default_rng(seed), n, layers)
@@ -99,7 +104,7 @@ def test_compiled_brickwork_against_oracle_and_reorder(n, layers, seed):
     g.run(arr, mode=RUN_REORDER)
     reo, passes_r = g.state(), g.run_stats()[1]
     g.close()
-    assert passes == passes_r
+    assert passes <= passes_r
     assert float(np.max(np.abs(comp - want))) <= 1e-12
     assert float(np.max(np.abs(comp - reo))) <= 1e-13
 
@@ -133,16 +138,16 @@ def test_compiled_kernels_are_reused_for_new_angles():
 
 
 def test_compiled_long_pass_runs_the_generic_kernel():
-    """A pass of more than 256 gates (X gates on qubit 0 under changing
-    controls all fit one tile) runs the generic kernel: same result as
-    KBWH_RUN_REORDER, one more pass counted as generic."""
+    """On 11 qubits a pass of more than 256 gates (RY gates on qubits 0..2
+    under changing controls all fit one tile) runs the generic kernel: same
+    result as KBWH_RUN_REORDER, one more pass counted as generic."""
     print("compiled long pass", flush=True)
-    n = 12
+    n = 11
     rng = np.random.default_rng(5)
     st = random_state(rng, n)
     ops = circuits.OpList()
     for i in range(300):
-        ops.add(6, i % 3, [3 + (i % 9)], angle=0.01 * i)
+        ops.add(6, i % 3, [3 + (i % 8)], angle=0.01 * i)
     arr = ops.array()
     before = compiled_stats()
     (comp, passes), (reo, _) = run_modes(n, st, arr, (RUN_COMPILED, RUN_REORDER))
@@ -150,3 +155,61 @@ def test_compiled_long_pass_runs_the_generic_kernel():
     assert passes == 1
     assert after["generic_passes"] - before["generic_passes"] == 1
     assert np.array_equal(comp, reo)
+
+
+def test_compiled_wide_long_pass_is_cut():
+    """On 12 qubits and more a pass of more than 256 gates is cut into passes
+    of at most 256, each compiled: 300 gates in one tile run as 2 compiled
+    passes, equal to KBWH_RUN_REORDER's single generic pass to rounding."""
+    print("compiled wide long pass", flush=True)
+    n = 12
+    rng = np.random.default_rng(6)
+    st = random_state(rng, n)
+    ops = circuits.OpList()
+    for i in range(300):
+        ops.add(6, i % 3, [3 + (i % 9)], angle=0.01 * i)
+    arr = ops.array()
+    before = compiled_stats()
+    (comp, passes), (reo, passes_r) = run_modes(n, st, arr, (RUN_COMPILED, RUN_REORDER))
+    after = compiled_stats()
+    assert (passes, passes_r) == (2, 1)
+    assert after["generic_passes"] == before["generic_passes"]
+    assert float(np.max(np.abs(comp - reo))) <= 1e-13
+
+
+@pytest.mark.parametrize("trials", [0, 64])
+def test_compiled_plan_search_and_cache(trials):
+    """The 30-qubit-shaped plan search on a 16-qubit brickwork: a searched
+    plan never has more passes than the greedy one, a second run of the same
+    structure with other angles reuses the plan and the kernels, and both
+    runs match the oracle to 1e-12."""
+    print(f"compiled plan search trials={trials}", flush=True)
+    n = 16
+    ops = circuits.OpList()
+    circuits.brickwork_layers(ops, np.random.default_rng(9), n, 8)
+    arr = ops.array()
+    g = DenseHip(n)
+    g.set_plan_search(0)
+    g.run(arr, mode=RUN_COMPILED)
+    greedy = g.run_stats()[1]
+    g.clear()
+    g.set_plan_search(trials)
+    g.run(arr, mode=RUN_COMPILED)
+    searched = g.run_stats()[1]
+    o = oracle.DenseV2(n)
+    o.run(arr)
+    assert float(np.max(np.abs(g.state() - o.state()))) <= 1e-12
+    assert searched <= greedy
+    arr2 = arr.copy()
+    rot = np.isin(arr2["kind"], [5, 6, 7])
+    arr2["angle"][rot] += 0.37
+    before = compiled_stats()
+    g.clear()
+    g.run(arr2, mode=RUN_COMPILED)
+    after = compiled_stats()
+    assert g.run_stats()[1] == searched
+    assert after["compiled"] == before["compiled"]
+    o2 = oracle.DenseV2(n)
+    o2.run(arr2)
+    assert float(np.max(np.abs(g.state() - o2.state()))) <= 1e-12
+    g.close()

@@ -601,11 +601,14 @@ def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12, compiled=Fa
     from |0>, the state is compared with the program-order tiled run over all
     2^n amplitudes (max |difference| <= tol), then K steps are timed.  The
     compiled mode's kernels are compiled by its first run (the comparison
-    run); their compile time is reported, not timed."""
+    run); their compile time is reported, not timed.  Its passes then try
+    their tile-order candidates (kbwh_set_order_tuning) in untimed runs until
+    every pass layout has chosen one; the order only permutes tiles, so the
+    compared state is the same bits whatever the order."""
     import numpy as np
     import torch
 
-    from kbw_amd import DenseHip, _lib, compiled_stats
+    from kbw_amd import DenseHip, _lib, compiled_stats, order_tuning_stats
 
     mode = _lib.RUN_COMPILED if compiled else _lib.RUN_REORDER
     nops = len(ops)
@@ -623,6 +626,12 @@ def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12, compiled=Fa
     worst = 0.0
     for first in range(0, 1 << n, chunk):
         worst = max(worst, float(np.max(np.abs(r.state(first, chunk) - sim.state(first, chunk)))))
+    tuning_runs = 0
+    if compiled:
+        # at most one run per candidate order (4) after the comparison run
+        while order_tuning_stats()["pending"] and tuning_runs < 8:
+            r.run(ops, mode=mode)
+            tuning_runs += 1
     for _ in range(args.warmup):
         r.run(ops, mode=mode)
     r.synchronize()
@@ -660,6 +669,8 @@ def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12, compiled=Fa
                           "generic_passes": after["generic_passes"] - before["generic_passes"],
                           "compile_s": after["compile_seconds"] - before["compile_seconds"],
                           "first_run_s": t_first}
+        out["order_tuning"] = dict(order_tuning_stats(), untimed_runs=tuning_runs,
+                                   candidates=[3, 5, 9, 0], default=3)
         out["within_tolerance"] = out["within_tolerance"] and out["compile"]["failures"] == 0
     return out
 

@@ -150,6 +150,20 @@ int32_t kbwh_set_plan_search(kbwh_state *s, int32_t trials);
  * the wall time spent compiling.  Any pointer may be NULL. */
 int32_t kbwh_compiled_stats(uint64_t *kernels_compiled, uint64_t *cache_hits, uint64_t *failures,
                             uint64_t *generic_passes, double *compile_seconds);
+/* Tile-order tuning of the 12-qubit compiled passes, for the process (no
+ * counterpart in the reference).  The order in which workgroups take a
+ * pass's tiles changes how its loads and stores fall on the HBM channels,
+ * not its results (every order is a permutation of independent tiles).  Each
+ * pass layout (kernel, tile qubits, tile count) of a state of 24 qubits or
+ * more tries 4 candidate orders on its first 4 runs, timed around its
+ * launch, and keeps the fastest.  Enabled by default; disabled, every pass
+ * runs the default order. */
+int32_t kbwh_set_order_tuning(int32_t enable);
+/* Pass layouts whose order is chosen, layouts still trying candidates,
+ * candidate launches timed so far, and chosen orders other than the default.
+ * Any pointer may be NULL. */
+int32_t kbwh_order_tuning_stats(uint64_t *layouts_tuned, uint64_t *layouts_pending, uint64_t *trials,
+                                uint64_t *non_default);
 /* The gfx950 code object of the specialised kernel of pass `pass` of the
  * KBWH_RUN_COMPILED plan of ops on a flat state of num_qubits (11..40)
  * qubits (12-qubit tiles from 12 qubits on, with the default plan search),

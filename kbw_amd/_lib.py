@@ -82,6 +82,8 @@ SIGNATURES = {
     "kbwh_set_compiled": [_vp, _i32],
     "kbwh_set_plan_search": [_vp, _i32],
     "kbwh_compiled_stats": [C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_f64)],
+    "kbwh_set_order_tuning": [_i32],
+    "kbwh_order_tuning_stats": [C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)],
     "kbwh_compiled_pass_code": [_p, _u64, _u32, _u32, _p, C.POINTER(_u64)],
     "kbwh_compiled_reverse_pass_code": [_p, _u64, _u32, _u32, _p, C.POINTER(_u64)],
     "kbwh_batch_run": [_vp, _p, _p],

@@ -143,9 +143,25 @@ int32_t jit_code(const JitPass &p, std::vector<char> &code);
 // set_mask: the amplitude mask of one set (kJitInit passes)
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
                       hipStream_t stream, uint32_t set_shift = 32, uint64_t set_mask = ~0ull);
-// a wide pass (spec: the 12-qubit tile::TileSpec of spec_bytes bytes)
+// a wide pass (spec: the 12-qubit tile::TileSpec of spec_bytes bytes) with
+// the tile order `order` (tile_of_block in tile_dev.h)
 hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates,
-                           uint32_t ntiles, hipStream_t stream);
+                           uint32_t ntiles, hipStream_t stream, uint32_t order);
+// Tile-order tuner for compiled passes (tile_jit.hip).  Each pass layout
+// (kernel, tile qubits, tile count) tries the candidate orders on its first
+// runs, one per run, timed by events around its launch, and then keeps the
+// fastest; the order only permutes which workgroup runs which tile.
+constexpr uint32_t kDefaultTileOrder = 3;
+struct OrderTrial {
+    void *slot;         // the layout's tuning record (null: no trial this run)
+    uint32_t order;     // the order to launch with
+    hipEvent_t ev[2];   // around the launch when slot is set
+};
+// the order for this launch; a trial when the layout is still being tuned
+// (the caller then records t.ev around the launch and reports the time)
+OrderTrial order_pick(int device, void *fn, const void *spec, size_t spec_bytes, uint32_t ntiles);
+// reports a trial after its launch completed (destroys its events)
+void order_report(OrderTrial &t);
 // Dependency-ordered runs in 12-qubit tiles with pass-specialised kernels
 // (tiled_wide.hip, KBWH_RUN_COMPILED on flat states of 12 qubits and more).
 // *done is false when the run was left to the 11-qubit passes (a pass kernel

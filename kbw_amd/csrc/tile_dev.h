@@ -214,20 +214,38 @@ __device__ __forceinline__ dv2 pair_val_lane(dv2 m, dv2 o, bool one, uint32_t sg
     }
 }
 
+// The in-tile controls of a pair gate: the record's cm, or CMT when the
+// kernel is compiled for its pass (CMT >= 0).  With the mask known at
+// compile time, the register-slot part of every control test is a constant:
+// slots whose register controls fail are left out altogether (no exchange,
+// no arithmetic, no select), and without thread controls no select remains.
+template <int64_t CMT>
+__device__ __forceinline__ uint32_t ctrl_mask(const TGate &G) {
+    if constexpr (CMT >= 0)
+        return (uint32_t)CMT;
+    else
+        return G.cm;
+}
+// slot r takes part in a gate whose register-slot controls are cm >> TB
+__device__ __forceinline__ constexpr bool slot_in(int r, uint32_t cm) {
+    return ((uint32_t)r & (cm >> TB)) == (cm >> TB);
+}
+
 // Target on register bit RB: both amplitudes of a pair live in one thread.
 // CTRL: some control lies inside the tile, so only the pairs whose index
 // matches cm change.
-template <int RB, int PK, bool CTRL>
+template <int RB, int PK, bool CTRL, int64_t CMT = -1>
 __device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, const TGate &G) {
     // An exchange has no instructions of its own: without this marker the
     // optimizer folds the register-bit variants of X into selects between
     // every candidate partner.
     if constexpr (PK == PK_X) asm volatile("");
-    const uint32_t cm = G.cm;
+    const uint32_t cm = ctrl_mask<CMT>(G);
     const bool tok = thread_ctrl_ok(tid, cm);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
         if ((r >> RB) & 1) continue;
+        if (CTRL && CMT >= 0 && !slot_in(r, cm)) continue;
         const int r1 = r | (1 << RB);
         const dv2 t0 = v[r], t1 = v[r1];
         const dv2 o0 = pair_val<PK>(t0, t1, false, G);
@@ -280,15 +298,16 @@ __device__ __forceinline__ double lane_xor_d(double x, int lm) {
     }
 }
 
-template <int PK, bool CTRL, int LX>
+template <int PK, bool CTRL, int LX, int64_t CMT = -1>
 __device__ __forceinline__ void lane_pair(dv2 (&v)[TR], uint32_t tid, uint32_t tp, const TGate &G) {
-    const uint32_t cm = G.cm;
+    const uint32_t cm = ctrl_mask<CMT>(G);
     const bool tok = thread_ctrl_ok(tid, cm);
     const bool one = ((tid >> tp) & 1u) != 0;
     const uint32_t sg = one ? 0x80000000u : 0u;
     const int lm = 1 << tp;
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
+        if (CTRL && CMT >= 0 && !slot_in(r, cm)) continue;
         const dv2 o = mk2(lane_xor_d<LX>(v[r].x, lm), lane_xor_d<LX>(v[r].y, lm));
         const dv2 out = pair_val_lane<PK>(v[r], o, one, sg, G);
         if constexpr (CTRL) {
@@ -344,20 +363,33 @@ __device__ __forceinline__ void thread_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid
 
 // The wave-bit part of thread_pair on its own, for the compile-time dispatch
 // (pair_at): target on wave bit tp (6 <= tp < TB).
-template <int PK, bool CTRL>
+template <int PK, bool CTRL, int64_t CMT = -1>
 __device__ __forceinline__ void wave_pair(dv2 (&v)[TR], dv2 *lds, uint32_t tid, uint32_t tp, const TGate &G) {
-    const uint32_t cm = G.cm;
+    const uint32_t cm = ctrl_mask<CMT>(G);
     const bool tok = thread_ctrl_ok(tid, cm);
     const bool one = ((tid >> tp) & 1u) != 0;
     const uint32_t pt = tid ^ (1u << tp);
+    // with the controls known, only the slots whose register controls hold
+    // go through LDS (a partner's slot r has the same register bits), and a
+    // chunk without such slots is skipped with its barriers (the same for
+    // every thread)
+    constexpr bool known = CTRL && CMT >= 0;
 #pragma unroll
     for (int q = 0; q < TR; q += kChunk) {
+        if (known) {
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < kChunk; ++r) any = any || slot_in(q + r, cm);
+            if (!any) continue;
+        }
         __syncthreads();  // earlier exchange reads are done
 #pragma unroll
-        for (int r = 0; r < kChunk; ++r) lds[tid + (r << TB)] = v[q + r];
+        for (int r = 0; r < kChunk; ++r)
+            if (!known || slot_in(q + r, cm)) lds[tid + (r << TB)] = v[q + r];
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < kChunk; ++r) {
+            if (known && !slot_in(q + r, cm)) continue;
             const dv2 o = lds[pt + (r << TB)];
             const dv2 out = pair_val_lane<PK>(v[q + r], o, one, one ? 0x80000000u : 0u, G);
             if constexpr (CTRL) {
@@ -586,11 +618,14 @@ __device__ __forceinline__ void swap_rows(double &p, double &q) {
     q = __longlong_as_double((long long)(((unsigned long long)qhi << 32) | qlo));
 }
 
-template <int LB, int K>
+// HI: register-slot controls of the gate (pairs whose controls fail are
+// neither changed by it nor moved)
+template <int LB, int K, uint32_t HI = 0>
 __device__ __forceinline__ void swap_lane_reg(dv2 (&v)[TR]) {
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
         if ((r >> K) & 1) continue;
+        if (((uint32_t)r & HI) != HI) continue;
         const int r1 = r | (1 << K);
         double ax = v[r].x, ay = v[r].y, bx = v[r1].x, by = v[r1].y;
         swap_rows<LB>(ax, bx);
@@ -617,17 +652,17 @@ constexpr int free_gate_reg_bit(uint32_t cm) {
 template <uint32_t TP, int PK, bool CTRL, uint32_t CM>
 __device__ __forceinline__ void pair_at(dv2 (&v)[TR], dv2 *lds, uint32_t tid, const TGate &G) {
     if constexpr (TP >= (uint32_t)TB) {
-        reg_pair<(int)(TP - TB), PK, CTRL>(v, tid, G);
+        reg_pair<(int)(TP - TB), PK, CTRL, CM>(v, tid, G);
     } else if constexpr ((TP == 4 || TP == 5) && free_gate_reg_bit(CM) >= 0) {
         constexpr int K = free_gate_reg_bit(CM);
-        swap_lane_reg<(int)TP, K>(v);
-        reg_pair<K, PK, CTRL>(v, tid, G);
-        swap_lane_reg<(int)TP, K>(v);
+        swap_lane_reg<(int)TP, K, (CM >> TB)>(v);
+        reg_pair<K, PK, CTRL, CM>(v, tid, G);
+        swap_lane_reg<(int)TP, K, (CM >> TB)>(v);
     } else if constexpr (TP < 6) {
         constexpr int LX = (TP & 4u) ? LX_SHFL : (TP & 2u) ? ((TP & 1u) ? LX_R8 : LX_R4) : ((TP & 1u) ? LX_Q2 : LX_Q1);
-        lane_pair<PK, CTRL, LX>(v, tid, TP, G);
+        lane_pair<PK, CTRL, LX, CM>(v, tid, TP, G);
     } else {
-        wave_pair<PK, CTRL>(v, lds, tid, TP, G);
+        wave_pair<PK, CTRL, CM>(v, lds, tid, TP, G);
     }
 }
 
@@ -713,6 +748,28 @@ __device__ __forceinline__ void apply_gate(dv2 (&v)[TR], dv2 *lds, uint64_t base
         else
             apply_pair(v, lds, tid, G);
     }
+}
+
+// The tile a workgroup takes.  Workgroups are dispatched in block order,
+// round-robin over the 8 XCDs, so the low bits of the block index vary
+// fastest over the workgroups in flight.  With ntiles a power of two, the
+// block index is rotated right by `order` bits within the tile index's
+// log2(ntiles) bits: those low bits become its top bits, and the workgroups
+// in flight spread over 2^order regions of the state (order 3: each XCD walks
+// its own eighth in tile order).  Which order streams best depends on the
+// pass's tile qubits, so it is measured per pass (the order tuner of
+// tile_jit.hip).  Other tile counts: order 3 with ntiles a multiple of 8
+// gives each XCD its own eighth, any other order is the block index.  Every
+// order is a permutation of the tiles, and each tile's arithmetic does not
+// depend on which workgroup runs it, so the results are the same bits.
+__device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t ntiles, uint32_t order) {
+    if ((ntiles & (ntiles - 1u)) == 0u) {
+        const uint32_t L = 31u - (uint32_t)__builtin_clz(ntiles | 1u);
+        if (order == 0u || order >= L) return b;
+        return ((b >> order) | (b << (L - order))) & (ntiles - 1u);
+    }
+    if (order == 3u && (ntiles & 7u) == 0u) return (b & 7u) * (ntiles >> 3) + (b >> 3);
+    return b;
 }
 
 // The thread's amplitude offsets in its tile: dt is the index of its slot 0.

@@ -121,12 +121,10 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
          "KBWH_TILE_WAVES))) void ";
     s += name;
     s += "(dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gp, uint32_t ntiles, uint32_t set_shift,\n"
-         "    uint64_t set_mask) {\n"
+         "    uint64_t set_mask, uint32_t order) {\n"
          "  __shared__ dv2 lds[TT * kChunk];\n"
          "  const uint32_t tid = threadIdx.x;\n"
-         // workgroups are dispatched round-robin over the 8 XCDs: those of one
-         // XCD take consecutive tiles, whose amplitudes share DRAM rows
-         "  const uint32_t tile = (ntiles & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (ntiles >> 3) + (blockIdx.x >> 3);\n"
+         "  const uint32_t tile = tile_of_block(blockIdx.x, ntiles, order);\n"
          "  if (tile >= ntiles) return;\n";
     s += "  if (set_shift < 32) gp += (size_t)(tile >> set_shift) * " + std::to_string(count) + "u;\n";
     s += "  const uint64_t base = tile_base(tile, ts);\n"
@@ -136,14 +134,21 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
     char line[320];
     for (uint32_t i = 0; i < count; ++i) {
         const TGate &G = g[i];
-        const int k = snprintf(line, sizeof line, "  { const TGate &G = gp[%u]; ", i);
+        // the thread index as a new value for every gate: the per-lane masks
+        // a gate derives from it are computed where the gate runs, not
+        // hoisted to the top of the kernel for every gate at once
+        const int k = snprintf(line, sizeof line,
+                               "  { uint32_t tg = tid; asm volatile(\"\" : \"+v\"(tg)); const TGate &G = gp[%u]; ", i);
         int m = k;
         if (G.ocm)
             m += snprintf(line + m, sizeof line - m, "if ((base & %lluull) == %lluull) ", (unsigned long long)G.ocm,
                           (unsigned long long)G.ocm);
-        snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tid, G); }\n", G.kind,
+        snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tg, G); }\n", G.kind,
                  G.tpos, G.cm, G.dlo, G.dhi);
         s += line;
+        // one gate at a time: scheduled across gates, long passes kept up to
+        // 198 registers live (2 waves per SIMD instead of 3)
+        s += "  __builtin_amdgcn_sched_barrier(0);\n";
     }
     s += "  tile_store(v, a, dt, ts);\n}\n";
     return s;
@@ -257,11 +262,23 @@ int64_t scratch_bytes(const std::vector<char> &code) {
 
 const char *const kOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
                                 "-Xclang", "-target-feature", "-Xclang",
-                                "-packed-fp32-ops"};
+                                "-packed-fp32-ops",
+                                // forward passes only (see compile_source)
+                                "-mllvm", "-structurizecfg-skip-uniform-regions=true"};
+constexpr int kNumOptions = (int)(sizeof kOptions / sizeof kOptions[0]);
 
 // compiles one pass kernel to a code object; false with a log on failure
-// (also when the kernel uses scratch memory, unless allow_scratch)
-bool compile_source(const std::string &src, std::vector<char> &code, std::string &log, bool allow_scratch = false) {
+// (also when the kernel uses scratch memory, unless allow_scratch).
+// uniform_regions: the kernel is straight-line code whose only branches are
+// uniform per-tile tests (a gate's controls outside the tile), which keep
+// their branch instead of being structurized into guarded blocks: guarded,
+// each such gate kept a second copy of the register tile live and long
+// passes spilled at 3 waves per SIMD.  The option mistreats a loop whose
+// lanes leave at different iterations followed by code that depends on how
+// it was left; a forward pass kernel has no loop left after unrolling, the
+// reverse sweep's kernels (a loop over tiles, block sums) are built without.
+bool compile_source(const std::string &src, std::vector<char> &code, std::string &log, bool allow_scratch = false,
+                    bool uniform_regions = false) {
     const Rtc &r = rtc();
     if (!r.ok) {
         log = r.why;
@@ -274,7 +291,7 @@ bool compile_source(const std::string &src, std::vector<char> &code, std::string
         log = "hiprtcCreateProgram failed";
         return false;
     }
-    const hiprtcResult rc = r.compile(p, (int)(sizeof kOptions / sizeof kOptions[0]), kOptions);
+    const hiprtcResult rc = r.compile(p, uniform_regions ? kNumOptions : kNumOptions - 2, kOptions);
     size_t n = 0;
     if (rc != HIPRTC_SUCCESS) {
         if (r.log_size(p, &n) == HIPRTC_SUCCESS && n > 1) {
@@ -313,9 +330,10 @@ bool compile_pass(const JitPass &p, const std::string &name, std::vector<char> &
     if (p.flags & kJitAdjoint)
         return compile_source(adj_pass_source((const AGateRec *)p.gates, p.count, name), code, log, allow_scratch);
     const TGate *g = (const TGate *)p.gates;
-    if (!(p.flags & kJitWide)) return compile_source(pass_source(g, p.count, p.flags, name), code, log, allow_scratch);
-    if (compile_source(pass_source(g, p.count, p.flags, name, 3), code, log, false)) return true;
-    return compile_source(pass_source(g, p.count, p.flags, name, 2), code, log, allow_scratch);
+    if (!(p.flags & kJitWide))
+        return compile_source(pass_source(g, p.count, p.flags, name), code, log, allow_scratch, true);
+    if (compile_source(pass_source(g, p.count, p.flags, name, 3), code, log, false, true)) return true;
+    return compile_source(pass_source(g, p.count, p.flags, name, 2), code, log, allow_scratch, true);
 }
 
 constexpr int kMaxDevices = 64;
@@ -436,12 +454,107 @@ hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gate
     memcpy(&ts, spec, sizeof ts);
     dv2 *a = (dv2 *)amp;
     const TGate *g = (const TGate *)gates;
-    void *args[] = {&a, &ts, &g, &ntiles, &set_shift, &set_mask};
+    uint32_t order = kDefaultTileOrder;
+    void *args[] = {&a, &ts, &g, &ntiles, &set_shift, &set_mask, &order};
     return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
 }
 
+// ---- tile-order tuner ----
+// Which tile order streams a pass best depends on its tile qubits (how the
+// tile's amplitudes and those of the tiles in flight beside it fall on the
+// HBM channels) and on how long its gates keep each workgroup: on the
+// 30-qubit brickwork circuit single passes moved from 10.1 to 7.2 ms and
+// from 9.3 to 8.2 ms between orders, others not at all, and no fixed order
+// was best for most passes.  So each pass layout measures the candidates on
+// its first runs (one per run, on the state it is run on: the order does not
+// change the results) and keeps the fastest.
+namespace {
+
+constexpr uint32_t kOrders[] = {3, 5, 9, 0};
+constexpr uint32_t kNumOrders = (uint32_t)(sizeof kOrders / sizeof kOrders[0]);
+// smaller states are not tuned (fewer tiles than a few waves of workgroups)
+constexpr uint32_t kMinTunedTiles = 1u << 12;
+
+struct OrderSlot {
+    float ms[kNumOrders];
+    uint32_t next = 0, reported = 0;
+    uint32_t best = kDefaultTileOrder;
+    OrderSlot() {
+        for (float &x : ms) x = -1.0f;
+    }
+};
+
+struct OrderTuner {
+    std::mutex mu;
+    std::map<std::string, OrderSlot> slots;
+    std::atomic<bool> enabled{true};
+    std::atomic<uint64_t> trials{0};
+};
+
+OrderTuner &tuner() {
+    static OrderTuner *t = new OrderTuner;  // kept until the process ends
+    return *t;
+}
+
+}  // namespace
+
+OrderTrial order_pick(int device, void *fn, const void *spec, size_t spec_bytes, uint32_t ntiles) {
+    OrderTrial t{nullptr, kDefaultTileOrder, {nullptr, nullptr}};
+    OrderTuner &T = tuner();
+    if (ntiles < kMinTunedTiles) return t;
+    std::string key;
+    key.append((const char *)&device, sizeof device);
+    key.append((const char *)&fn, sizeof fn);
+    key.append((const char *)&ntiles, sizeof ntiles);
+    key.append((const char *)spec, spec_bytes);
+    std::lock_guard<std::mutex> lk(T.mu);
+    OrderSlot &o = T.slots[key];
+    if (!T.enabled.load() || o.next >= kNumOrders) {
+        t.order = T.enabled.load() ? o.best : kDefaultTileOrder;
+        return t;
+    }
+    if (hipEventCreate(&t.ev[0]) != hipSuccess) {
+        (void)hipGetLastError();
+        return t;
+    }
+    if (hipEventCreate(&t.ev[1]) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipEventDestroy(t.ev[0]);
+        t.ev[0] = nullptr;
+        return t;
+    }
+    t.slot = &o;  // map nodes stay where they are
+    t.order = kOrders[o.next++];
+    T.trials++;
+    return t;
+}
+
+void order_report(OrderTrial &t) {
+    if (!t.slot) return;
+    float ms = -1.0f;
+    if (hipEventElapsedTime(&ms, t.ev[0], t.ev[1]) != hipSuccess) {
+        (void)hipGetLastError();
+        ms = -1.0f;
+    }
+    (void)hipEventDestroy(t.ev[0]);
+    (void)hipEventDestroy(t.ev[1]);
+    OrderTuner &T = tuner();
+    std::lock_guard<std::mutex> lk(T.mu);
+    OrderSlot &o = *(OrderSlot *)t.slot;
+    t.slot = nullptr;
+    uint32_t idx = 0;
+    while (idx < kNumOrders && kOrders[idx] != t.order) ++idx;
+    if (idx == kNumOrders || ms < 0.0f) return;
+    o.ms[idx] = ms;
+    if (++o.reported < kNumOrders) return;
+    uint32_t b = 0;  // ties keep the earlier candidate (the default first)
+    for (uint32_t i = 1; i < kNumOrders; ++i)
+        if (o.ms[i] >= 0.0f && (o.ms[b] < 0.0f || o.ms[i] < o.ms[b])) b = i;
+    o.best = kOrders[b];
+}
+
 hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates,
-                           uint32_t ntiles, hipStream_t stream) {
+                           uint32_t ntiles, hipStream_t stream, uint32_t order) {
     alignas(8) unsigned char ts[256];
     if (spec_bytes > sizeof ts) return hipErrorInvalidValue;
     memcpy(ts, spec, spec_bytes);
@@ -449,7 +562,7 @@ hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec
     const void *g = gates;
     uint32_t set_shift = 32;
     uint64_t set_mask = ~0ull;
-    void *args[] = {&a, ts, &g, &ntiles, &set_shift, &set_mask};
+    void *args[] = {&a, ts, &g, &ntiles, &set_shift, &set_mask, &order};
     return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
 }
 
@@ -472,6 +585,32 @@ hipError_t jit_launch_adj(void *fn, double2 *amp, const void *spec, size_t spec_
 }
 
 }  // namespace kbwh
+
+extern "C" int32_t kbwh_set_order_tuning(int32_t enable) {
+    kbwh::tuner().enabled = enable != 0;
+    return KBWH_OK;
+}
+
+extern "C" int32_t kbwh_order_tuning_stats(uint64_t *layouts_tuned, uint64_t *layouts_pending, uint64_t *trials,
+                                           uint64_t *non_default) {
+    using namespace kbwh;
+    OrderTuner &T = tuner();
+    std::lock_guard<std::mutex> lk(T.mu);
+    uint64_t tuned = 0, pending = 0, moved = 0;
+    for (const auto &kv : T.slots) {
+        if (kv.second.reported >= kNumOrders) {
+            ++tuned;
+            moved += kv.second.best != kDefaultTileOrder;
+        } else {
+            ++pending;
+        }
+    }
+    if (layouts_tuned) *layouts_tuned = tuned;
+    if (layouts_pending) *layouts_pending = pending;
+    if (trials) *trials = T.trials.load();
+    if (non_default) *non_default = moved;
+    return KBWH_OK;
+}
 
 extern "C" int32_t kbwh_compiled_stats(uint64_t *kernels_compiled, uint64_t *cache_hits, uint64_t *failures,
                                        uint64_t *generic_passes, double *compile_seconds) {

@@ -130,13 +130,33 @@ int32_t run_compiled_wide(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint6
     if (e != hipSuccess) return hip_status(e, "tile gate upload");
     const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
     const TGate *dg = (const TGate *)s->tile_gates;
-    for (size_t i = 0; i < passes.size(); ++i) {
-        e = jit_launch_spec(fns[i], amp, &passes[i].spec, sizeof(TileSpec), dg + passes[i].first, ntiles, s->stream);
-        if (e != hipSuccess) return hip_status(e, "wide pass launch");
+    std::vector<OrderTrial> trials;
+    trials.reserve(passes.size());
+    for (size_t i = 0; i < passes.size() && e == hipSuccess; ++i) {
+        trials.push_back(order_pick(s->device, fns[i], &passes[i].spec, sizeof(TileSpec), ntiles));
+        OrderTrial &t = trials.back();
+        if (t.slot) e = hipEventRecord(t.ev[0], s->stream);
+        if (e == hipSuccess)
+            e = jit_launch_spec(fns[i], amp, &passes[i].spec, sizeof(TileSpec), dg + passes[i].first, ntiles,
+                                s->stream, t.order);
+        if (e == hipSuccess && t.slot) e = hipEventRecord(t.ev[1], s->stream);
     }
-    s->last_passes = passes.size();
     // the host records are local: the upload must finish before they go away
-    return hip_status(hipStreamSynchronize(s->stream), "wide passes");
+    const hipError_t se = hipStreamSynchronize(s->stream);
+    for (OrderTrial &t : trials) {
+        if (e != hipSuccess || se != hipSuccess) {
+            // a failed run reports no times; its trials' events go away
+            if (t.slot) {
+                (void)hipEventDestroy(t.ev[0]);
+                (void)hipEventDestroy(t.ev[1]);
+            }
+            continue;
+        }
+        order_report(t);
+    }
+    if (e != hipSuccess) return hip_status(e, "wide pass launch");
+    s->last_passes = passes.size();
+    return hip_status(se, "wide passes");
 }
 
 int32_t wide_pass_records(const kbwh_op *ops, uint64_t num_ops, uint32_t num_qubits, uint32_t pass,

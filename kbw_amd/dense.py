@@ -98,6 +98,21 @@ def compiled_stats():
             "generic_passes": v[3].value, "compile_seconds": sec.value}
 
 
+def set_order_tuning(enable=True):
+    """Tile-order tuning of the 12-qubit compiled passes for the process
+    (kbwh_set_order_tuning): each pass layout tries the candidate orders on
+    its first runs and keeps the fastest.  The order does not change results."""
+    check(_lib.load().kbwh_set_order_tuning(1 if enable else 0))
+
+
+def order_tuning_stats():
+    """A dict of pass layouts tuned, layouts still trying orders, candidate
+    launches timed and chosen orders other than the default."""
+    v = [C.c_uint64() for _ in range(4)]
+    check(_lib.load().kbwh_order_tuning_stats(*[C.byref(x) for x in v]))
+    return {"tuned": v[0].value, "pending": v[1].value, "trials": v[2].value, "non_default": v[3].value}
+
+
 def compiled_pass_code(ops, num_qubits, pass_index, reverse=False):
     """The gfx950 code object (bytes) of the RUN_COMPILED kernel of one pass
     (reverse=True: of the adjoint's reverse sweep), compiled on the host."""

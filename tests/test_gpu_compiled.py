@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 import oracle
-from kbw_amd import (DenseHip, RUN_COMPILED, RUN_PER_GATE, RUN_REORDER, circuits, compiled_stats)
+from kbw_amd import (DenseHip, RUN_COMPILED, RUN_PER_GATE, RUN_REORDER, circuits, compiled_stats, order_tuning_stats,
+                     set_order_tuning)
 
 pytestmark = pytest.mark.gpu
 
@@ -212,4 +213,41 @@ def test_compiled_plan_search_and_cache(trials):
     o2 = oracle.DenseV2(n)
     o2.run(arr2)
     assert float(np.max(np.abs(g.state() - o2.state()))) <= 1e-12
+    g.close()
+
+
+def test_compiled_tile_order_tuning_keeps_the_bits():
+    """The tile-order tuner (24 qubits: 4096 tiles per pass): the first runs
+    of each pass layout launch it with each candidate order, then the fastest
+    is kept.  The order only permutes which workgroup runs which tile, so
+    every run from the same state gives the same bits; after the candidate
+    runs every layout of the circuit is tuned, and the state matches the
+    oracle's program-order run to 1e-12.  With tuning off the bits are the
+    same again."""
+    print("compiled tile-order tuning 24q", flush=True)
+    n = 24
+    ops = circuits.OpList()
+    circuits.brickwork_layers(ops, np.random.default_rng(21), n, 6)
+    arr = ops.array()
+    g = DenseHip(n)
+    states = []
+    for _ in range(6):
+        g.clear()
+        g.run(arr, mode=RUN_COMPILED)
+        states.append(g.state())
+    passes = g.run_stats()[1]
+    st = order_tuning_stats()
+    assert st["pending"] == 0 and st["tuned"] >= passes and st["trials"] >= 4 * passes, st
+    for s in states[1:]:
+        assert np.array_equal(s, states[0])
+    set_order_tuning(False)
+    try:
+        g.clear()
+        g.run(arr, mode=RUN_COMPILED)
+        assert np.array_equal(g.state(), states[0])
+    finally:
+        set_order_tuning(True)
+    o = oracle.DenseV2(n)
+    o.run(arr)
+    assert float(np.max(np.abs(states[0] - o.state()))) <= 1e-12
     g.close()

@@ -226,6 +226,13 @@ __device__ __forceinline__ uint32_t ctrl_mask(const TGate &G) {
     else
         return G.cm;
 }
+// Compiled kernels: a gate's slots in groups of 4.  Scheduled across a whole
+// tile, the 16 slots' products and exchanges kept their temporaries live at
+// once (up to 132 registers where 128 give 4 waves per SIMD).
+__device__ __forceinline__ void slot_group(int r) {
+    if (r > 0 && (r & 3) == 0) __builtin_amdgcn_sched_barrier(0);
+}
+
 // slot r takes part in a gate whose register-slot controls are cm >> TB
 __device__ __forceinline__ constexpr bool slot_in(int r, uint32_t cm) {
     return ((uint32_t)r & (cm >> TB)) == (cm >> TB);
@@ -244,6 +251,7 @@ __device__ __forceinline__ void reg_pair(dv2 (&v)[TR], uint32_t tid, const TGate
     const bool tok = thread_ctrl_ok(tid, cm);
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
+        if constexpr (CMT >= 0) slot_group(r);
         if ((r >> RB) & 1) continue;
         if (CTRL && CMT >= 0 && !slot_in(r, cm)) continue;
         const int r1 = r | (1 << RB);
@@ -307,6 +315,7 @@ __device__ __forceinline__ void lane_pair(dv2 (&v)[TR], uint32_t tid, uint32_t t
     const int lm = 1 << tp;
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
+        if constexpr (CMT >= 0) slot_group(r);
         if (CTRL && CMT >= 0 && !slot_in(r, cm)) continue;
         const dv2 o = mk2(lane_xor_d<LX>(v[r].x, lm), lane_xor_d<LX>(v[r].y, lm));
         const dv2 out = pair_val_lane<PK>(v[r], o, one, sg, G);
@@ -668,24 +677,31 @@ __device__ __forceinline__ void pair_at(dv2 (&v)[TR], dv2 *lds, uint32_t tid, co
 
 // apply_diag_t with its runtime tests resolved: DK, target position TP, cm,
 // and the record's thread / register-slot masks DLO / DHI
-template <int DK, uint32_t TP, uint32_t CM, uint32_t DLO, uint32_t DHI>
+// OUT: the record's outside bits (G.ocm) are tested here rather than by a
+// branch around the gate (Z: the test joins the per-thread sign mask, so a
+// tile that is not selected flips nothing; see z_outside_at).
+template <int DK, uint32_t TP, uint32_t CM, uint32_t DLO, uint32_t DHI, bool OUT = false>
 __device__ __forceinline__ void diag_at(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
     const dv2 ph0 = mk2(G.ph0x, G.ph0y), ph1 = mk2(G.ph1x, G.ph1y);
     constexpr bool reg_t = TP >= (uint32_t)TB && TP != kOutside;
     constexpr bool out_t = TP == kOutside;
     constexpr bool thr_t = !reg_t && !out_t;
     if constexpr (DK == DK_Z || DK == DK_P) {
-        const bool sel = (tid & DLO) == DLO;
+        const bool sel = (tid & DLO) == DLO && (!OUT || (base & G.ocm) == G.ocm);
         if constexpr (DK == DK_Z) {
             const uint32_t m = sel ? 0x80000000u : 0u;
 #pragma unroll
-            for (int r = 0; r < TR; ++r)
+            for (int r = 0; r < TR; ++r) {
+                slot_group(r);
                 if (((uint32_t)r & DHI) == DHI) v[r] = flip_sign(v[r], m);
+            }
         } else {
             const dv2 f = sel ? ph1 : mk2(1.0, 0.0);
 #pragma unroll
-            for (int r = 0; r < TR; ++r)
+            for (int r = 0; r < TR; ++r) {
+                slot_group(r);
                 if (((uint32_t)r & DHI) == DHI) v[r] = cmul(v[r], f);
+            }
         }
     } else {
         const bool ok = (tid & DLO) == DLO;
@@ -697,21 +713,69 @@ __device__ __forceinline__ void diag_at(dv2 (&v)[TR], uint64_t base, uint32_t ti
                 one = (base & G.otb) != 0;
             const dv2 f = ok ? (one ? ph1 : ph0) : mk2(1.0, 0.0);
 #pragma unroll
-            for (int r = 0; r < TR; ++r)
+            for (int r = 0; r < TR; ++r) {
+                slot_group(r);
                 if (((uint32_t)r & DHI) == DHI) v[r] = cmul(v[r], f);
+            }
         } else if constexpr (CM == 0) {
             constexpr uint32_t TM = 1u << (TP - TB);
 #pragma unroll
-            for (int r = 0; r < TR; ++r) v[r] = cmul(v[r], ((uint32_t)r & TM) ? ph1 : ph0);
+            for (int r = 0; r < TR; ++r) {
+                slot_group(r);
+                v[r] = cmul(v[r], ((uint32_t)r & TM) ? ph1 : ph0);
+            }
         } else {
             constexpr uint32_t rb = TP - TB;
 #pragma unroll
             for (int r = 0; r < TR; ++r) {
+                slot_group(r);
                 const dv2 ph = ((uint32_t)r & DHI) == DHI ? ((((uint32_t)r >> rb) & 1u) ? ph1 : ph0) : mk2(1.0, 0.0);
                 v[r] = cmul(v[r], ok ? ph : mk2(1.0, 0.0));
             }
         }
     }
+}
+
+// A diagonal gate whose target and controls all lie outside the tile
+// multiplies the whole tile by one phase (fixed by the tile's outside bits),
+// and a tile-wide scalar commutes with every gate of the pass.  Compiled
+// passes therefore collect these gates into one per-tile phase ph (products
+// of the records' phases, computed without branches) and apply it once,
+// before the store, where it is not exactly 1.  Z and P: the phase of the
+// |1> half where the target and outside controls (G.ocm, which holds the
+// target) are all 1; RZ: the phase of the target's half where the outside
+// controls hold.
+template <uint32_t KIND>
+__device__ __forceinline__ void tile_phase_acc(dv2 &ph, uint64_t base, const TGate &G) {
+    dv2 f;
+    if constexpr (KIND == KRZ) {
+        const bool ok = (base & G.ocm) == G.ocm;
+        const bool one = (base & G.otb) != 0;
+        f = ok ? (one ? mk2(G.ph1x, G.ph1y) : mk2(G.ph0x, G.ph0y)) : mk2(1.0, 0.0);
+    } else {
+        f = (base & G.ocm) == G.ocm ? mk2(G.ph1x, G.ph1y) : mk2(1.0, 0.0);
+    }
+    ph = cmul(ph, f);
+}
+
+__device__ __forceinline__ void tile_phase_apply(dv2 (&v)[TR], dv2 ph) {
+    if (ph.x != 1.0 || ph.y != 0.0) {
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+            slot_group(r);
+            v[r] = cmul(v[r], ph);
+        }
+    }
+}
+
+// A Z with outside bits (controls, or its target) and the rest in the tile:
+// without a branch around it.  A per-tile branch around a gate makes the
+// register tile a join of two versions, and a pass with many of them kept
+// about 15 more registers live (a long pass then spilled at 4 waves per
+// SIMD); a Z costs a sign flip either way.
+template <uint32_t TP, uint32_t CM, uint32_t DLO, uint32_t DHI>
+__device__ __forceinline__ void z_outside_at(dv2 (&v)[TR], uint64_t base, uint32_t tid, const TGate &G) {
+    diag_at<DK_Z, TP, CM, DLO, DHI, true>(v, base, tid, G);
 }
 
 // One gate whose kind, target position, in-tile controls and diagonal masks

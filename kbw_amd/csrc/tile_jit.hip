@@ -132,8 +132,29 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
          "  dv2 v[TR];\n";
     s += (flags & kJitInit) ? "  tile_zero_state(v, dt, ts, set_mask);\n" : "  tile_load(v, a, dt, ts);\n";
     char line[320];
+    // diagonal gates wholly outside the tile: one per-tile phase (tile_phase_acc)
+    auto tile_wide = [](const TGate &G) {
+        return (G.kind == KZ || G.kind == KP || G.kind == KRZ) && G.tpos == kOutside && G.cm == 0;
+    };
+    bool any_phase = false;
+    for (uint32_t i = 0; i < count; ++i) any_phase = any_phase || tile_wide(g[i]);
+    if (any_phase) s += "  dv2 ph = mk2(1.0, 0.0);\n";
     for (uint32_t i = 0; i < count; ++i) {
         const TGate &G = g[i];
+        if (tile_wide(G)) {
+            snprintf(line, sizeof line, "  tile_phase_acc<%uu>(ph, base, gp[%u]);\n", G.kind, i);
+            s += line;
+            continue;
+        }
+        if (G.kind == KZ && G.ocm) {
+            snprintf(line, sizeof line,
+                     "  { uint32_t tg = tid; asm volatile(\"\" : \"+v\"(tg)); "
+                     "z_outside_at<%uu, %uu, %uu, %uu>(v, base, tg, gp[%u]); }\n"
+                     "  __builtin_amdgcn_sched_barrier(0);\n",
+                     G.tpos, G.cm, G.dlo, G.dhi, i);
+            s += line;
+            continue;
+        }
         // the thread index as a new value for every gate: the per-lane masks
         // a gate derives from it are computed where the gate runs, not
         // hoisted to the top of the kernel for every gate at once
@@ -150,6 +171,7 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
         // 198 registers live (2 waves per SIMD instead of 3)
         s += "  __builtin_amdgcn_sched_barrier(0);\n";
     }
+    if (any_phase) s += "  tile_phase_apply(v, ph);\n";
     s += "  tile_store(v, a, dt, ts);\n}\n";
     return s;
 }
@@ -332,6 +354,7 @@ bool compile_pass(const JitPass &p, const std::string &name, std::vector<char> &
     const TGate *g = (const TGate *)p.gates;
     if (!(p.flags & kJitWide))
         return compile_source(pass_source(g, p.count, p.flags, name), code, log, allow_scratch, true);
+    if (compile_source(pass_source(g, p.count, p.flags, name, 4), code, log, false, true)) return true;
     if (compile_source(pass_source(g, p.count, p.flags, name, 3), code, log, false, true)) return true;
     return compile_source(pass_source(g, p.count, p.flags, name, 2), code, log, allow_scratch, true);
 }

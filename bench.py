@@ -50,7 +50,7 @@ HBM_PEAK_GBPS = 8000.0
 TRAFFIC_PAIR = 34362315175.0
 TRAFFIC_TILE = 34369246667.0
 TRAFFIC_TILE_FMA = 34359974941.0
-TRAFFIC_COMPILED = 34363556768.0
+TRAFFIC_COMPILED = 34365480930.0
 
 
 def parse():

@@ -644,6 +644,46 @@ __device__ __forceinline__ void swap_lane_reg(dv2 (&v)[TR]) {
     }
 }
 
+// slot index j with a 0 bit inserted at bit K
+__device__ __forceinline__ constexpr int insert_zero_bit(int j, int K) {
+    return ((j >> K) << (K + 1)) | (j & ((1 << K) - 1));
+}
+
+// Exchange of wave bit W with register bit K through LDS: the amplitude at
+// wave bit W = x and register bit K = y moves to wave bit y and register bit
+// x; every other bit stays.  Of each register pair (r, r | 1 << K) a thread
+// keeps the slot whose bit K equals its wave bit and trades the other with
+// its partner across W, so half of a wave-bit gate's LDS bytes move.
+template <int W, int K>
+__device__ __forceinline__ void swap_wave_reg(dv2 (&v)[TR], dv2 *lds, uint32_t tid) {
+    const bool hi = ((tid >> W) & 1u) != 0;
+    const uint32_t pt = tid ^ (1u << W);
+    constexpr int NP = TR / 2;
+    static_assert(NP % kChunk == 0, "register pairs per LDS round");
+#pragma unroll
+    for (int q = 0; q < NP; q += kChunk) {
+        __syncthreads();  // earlier exchange reads are done
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+            const int r0 = insert_zero_bit(q + c, K), r1 = r0 | (1 << K);
+            // values, not array elements: a select between two elements of
+            // the register array becomes a select between their addresses
+            // and puts the array in scratch memory
+            dv2 x0 = v[r0], x1 = v[r1];
+            asm volatile("" : "+v"(x0), "+v"(x1));
+            lds[tid + (c << TB)] = hi ? x0 : x1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+            const int r0 = insert_zero_bit(q + c, K), r1 = r0 | (1 << K);
+            const dv2 in = lds[pt + (c << TB)];
+            v[r0] = hi ? in : v[r0];
+            v[r1] = hi ? v[r1] : in;
+        }
+    }
+}
+
 // A gate register bit that is not among the in-tile controls CM (-1: none)
 constexpr int free_gate_reg_bit(uint32_t cm) {
     for (int k = TGR - 1; k >= 0; --k)
@@ -848,6 +888,28 @@ __device__ __forceinline__ uint64_t thread_offset(uint64_t base, uint32_t tid, c
 __device__ __forceinline__ void tile_load(dv2 (&v)[TR], const dv2 *__restrict__ a, uint64_t dt, const TileSpec &ts) {
 #pragma unroll
     for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));
+}
+
+// tile_store through a permuted layout: position p holds the qubit loaded at
+// position AT[p] (a compiled pass that moved qubits between positions).
+template <int... AT>
+__device__ __forceinline__ void tile_store_at(const dv2 (&v)[TR], dv2 *__restrict__ a, uint64_t base, uint32_t tid,
+                                              const TileSpec &ts) {
+    static_assert(sizeof...(AT) == TL, "one load position per tile position");
+    constexpr int at[TL] = {AT...};
+    uint64_t dt = base;
+#pragma unroll
+    for (int b = 0; b < TB; ++b)
+        if ((tid >> b) & 1u) dt |= ts.qbit[at[b]];
+    asm volatile("" : "+v"(dt));
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        uint64_t o = dt;
+#pragma unroll
+        for (int k = 0; k < TL - TB; ++k)
+            if ((r >> k) & 1) o |= ts.qbit[at[TB + k]];
+        __builtin_nontemporal_store(v[r], a + o);
+    }
 }
 
 // |0...0> of every set in registers: amplitude 1 where the index within its

@@ -136,6 +136,36 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
     auto tile_wide = [](const TGate &G) {
         return (G.kind == KZ || G.kind == KP || G.kind == KRZ) && G.tpos == kOutside && G.cm == 0;
     };
+    // Layout of a wide pass, followed through the pass (the records hold load
+    // positions): a pair gate whose target sits on a wave bit or on lane bit
+    // 4 or 5 first trades that position with a register bit (swap_wave_reg
+    // through LDS, half of a wave-bit gate's LDS bytes; swap_lane_reg by
+    // permlane swaps) and runs as a register-bit gate, and the qubit stays
+    // there for its later gates.  The register bit given up is the one whose
+    // qubit is next a target furthest ahead.  cur[l]: the position now
+    // holding the qubit loaded at position l; at[p]: the load position of
+    // the qubit now at p.  The tile is stored through the final layout.
+    const bool relayout = (flags & kJitWide) != 0;
+    constexpr int kTb = 8;
+    const int tl = (flags & kJitWide) ? 12 : 11;
+    int cur[16], at[16];
+    for (int q = 0; q < 16; ++q) cur[q] = at[q] = q;
+    bool moved = false;
+    auto is_diag = [](const TGate &G) { return G.kind == KZ || G.kind == KP || G.kind == KRZ; };
+    auto remap = [&](uint32_t m) {
+        uint32_t o = 0;
+        for (int q = 0; q < tl; ++q)
+            if ((m >> q) & 1u) o |= 1u << cur[q];
+        return o;
+    };
+    // positions of gate G in the current layout: target, in-tile controls, diagonal masks
+    auto positions = [&](const TGate &G, uint32_t &tp, uint32_t &cm, uint32_t &dlo, uint32_t &dhi) {
+        tp = G.tpos == kOutside ? kOutside : (uint32_t)cur[G.tpos];
+        cm = remap(G.cm);
+        const uint32_t d = remap(G.dlo | (G.dhi << kTb));
+        dlo = d & ((1u << kTb) - 1u);
+        dhi = d >> kTb;
+    };
     bool any_phase = false;
     for (uint32_t i = 0; i < count; ++i) any_phase = any_phase || tile_wide(g[i]);
     if (any_phase) s += "  dv2 ph = mk2(1.0, 0.0);\n";
@@ -146,12 +176,47 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
             s += line;
             continue;
         }
+        if (relayout && !is_diag(G) && G.tpos != kOutside) {
+            const int p = cur[G.tpos];
+            const bool wave = p >= 6 && p < kTb, lane45 = p == 4 || p == 5;
+            if (wave || lane45) {
+                // the register bit whose qubit is next a target furthest ahead
+                int best = kTb;
+                uint32_t best_next = 0;
+                for (int r = kTb; r < tl; ++r) {
+                    uint32_t nx = count;
+                    for (uint32_t j = i + 1; j < count; ++j)
+                        if (!is_diag(g[j]) && g[j].tpos == (uint32_t)at[r]) {
+                            nx = j;
+                            break;
+                        }
+                    if (r == kTb || nx > best_next) {
+                        best = r;
+                        best_next = nx;
+                    }
+                }
+                if (wave)
+                    snprintf(line, sizeof line, "  swap_wave_reg<%d, %d>(v, lds, tid);\n", p, best - kTb);
+                else
+                    snprintf(line, sizeof line, "  swap_lane_reg<%d, %d>(v);\n", p, best - kTb);
+                s += line;
+                s += "  __builtin_amdgcn_sched_barrier(0);\n";
+                const int a = at[p], b = at[best];
+                at[p] = b;
+                at[best] = a;
+                cur[a] = best;
+                cur[b] = p;
+                moved = true;
+            }
+        }
+        uint32_t tp, cm, dlo, dhi;
+        positions(G, tp, cm, dlo, dhi);
         if (G.kind == KZ && G.ocm) {
             snprintf(line, sizeof line,
                      "  { uint32_t tg = tid; asm volatile(\"\" : \"+v\"(tg)); "
                      "z_outside_at<%uu, %uu, %uu, %uu>(v, base, tg, gp[%u]); }\n"
                      "  __builtin_amdgcn_sched_barrier(0);\n",
-                     G.tpos, G.cm, G.dlo, G.dhi, i);
+                     tp, cm, dlo, dhi, i);
             s += line;
             continue;
         }
@@ -165,14 +230,21 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
             m += snprintf(line + m, sizeof line - m, "if ((base & %lluull) == %lluull) ", (unsigned long long)G.ocm,
                           (unsigned long long)G.ocm);
         snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tg, G); }\n", G.kind,
-                 G.tpos, G.cm, G.dlo, G.dhi);
+                 tp, cm, dlo, dhi);
         s += line;
         // one gate at a time: scheduled across gates, long passes kept up to
         // 198 registers live (2 waves per SIMD instead of 3)
         s += "  __builtin_amdgcn_sched_barrier(0);\n";
     }
     if (any_phase) s += "  tile_phase_apply(v, ph);\n";
-    s += "  tile_store(v, a, dt, ts);\n}\n";
+    if (moved) {
+        // the store through the final layout: position p holds the qubit loaded at at[p]
+        s += "  tile_store_at<";
+        for (int q = 0; q < tl; ++q) s += std::to_string(at[q]) + (q + 1 < tl ? ", " : "");
+        s += ">(v, a, base, tid, ts);\n}\n";
+    } else {
+        s += "  tile_store(v, a, dt, ts);\n}\n";
+    }
     return s;
 }
 

@@ -132,8 +132,11 @@ int32_t kbwh_set_run_config(kbwh_state *s, uint32_t fusion_width, uint32_t tile_
 
 /* enable != 0: the handle's adjoint gradients (kbwh_adjoint_gradient,
  * kbwh_adjoint_batch) run their forward and reverse passes on kernels compiled
- * for each pass, as KBWH_RUN_COMPILED (no counterpart in the reference).  The
- * first call of a circuit structure compiles them; values agree with the
+ * for each pass, as KBWH_RUN_COMPILED, and kbwh_exp_value's grouped sweeps
+ * (flat states of 11 qubits or more) run kernels compiled for each group of
+ * strings (no counterpart in the reference; kbwh_exp_value_noalloc keeps the
+ * generic sweep, since loading a kernel allocates).  The first call of a
+ * circuit or string-group structure compiles them; values agree with the
  * generic kernels to rounding. */
 int32_t kbwh_set_compiled(kbwh_state *s, int32_t enable);
 /* Plan search of the dependency-ordered runs (KBWH_RUN_REORDER,

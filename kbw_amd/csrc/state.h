@@ -131,6 +131,12 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
 constexpr uint32_t kJitInit = 1;     // the pass starts from |0...0> of each set instead of a load
 constexpr uint32_t kJitAdjoint = 2;  // a reverse-sweep pass of the adjoint (records: the sweep's AGate)
 constexpr uint32_t kJitWide = 4;     // 12-qubit tiles (records and TileSpec of the wide planner)
+constexpr uint32_t kJitSweep = 8;    // an exp_value sweep of one string group (records: SweepKey)
+// the compile-time part of one string of a sweep: x as tile positions, z on
+// the register positions, ny mod 4
+struct SweepKey {
+    uint32_t xt, zr, q, pad;
+};
 struct JitPass {
     const void *gates;  // host tile::TGate records of the pass (one set)
     uint32_t count;     // 0: no specialised kernel
@@ -143,6 +149,10 @@ int32_t jit_code(const JitPass &p, std::vector<char> &code);
 // set_mask: the amplitude mask of one set (kJitInit passes)
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
                       hipStream_t stream, uint32_t set_shift = 32, uint64_t set_mask = ~0ull);
+// an exp_value sweep (11-qubit tiles): strings = the group's device string
+// records; one partial sum per workgroup into partial[0..grid)
+hipError_t jit_launch_sweep(void *fn, const double2 *amp, const void *spec, const void *strings, uint32_t ntiles,
+                            double *partial, uint32_t grid, hipStream_t stream);
 // a wide pass (spec: the 12-qubit tile::TileSpec of spec_bytes bytes) with
 // the tile order `order` (tile_of_block in tile_dev.h)
 hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates,

@@ -927,6 +927,93 @@ __device__ __forceinline__ void tile_store(const dv2 (&v)[TR], dv2 *__restrict__
     for (int r = 0; r < TR; ++r) __builtin_nontemporal_store(v[r], a + slot_offset(dt, r, ts));
 }
 
+// ---- Pauli-string sweeps compiled per string group (exp_value) ----
+// A string P = i^ny X^x Z^z gives <psi|P|psi> = sum_i conj(a_{i^x}) a_i
+// i^ny (-1)^popcount(i & z).  Per tile, w_i = conj(a_{i^x}) a_i is formed
+// once for every string with the same x (the partner comes from a register,
+// a lane move or the tile in LDS, chosen at compile time), and each string
+// adds the real part of i^ny w_i with the sign of popcount(i & z): the
+// register-slot part of that parity and i^ny are compile-time constants, the
+// part from the tile's and thread's bits one test per string and tile.
+
+// a string record of a sweep (tiled.hip's string table): z as global bits,
+// ny, x as tile positions and the string's coefficient
+struct SweepString {
+    uint64_t z;
+    uint32_t ny, xt;
+    double coeff;
+};
+static_assert(sizeof(SweepString) == 24, "sweep string record");
+
+// lane xor by a compile-time mask: one DPP move for 1, 2 and 8, two for 4,
+// a cross-lane shuffle otherwise
+template <uint32_t XL>
+__device__ __forceinline__ double lane_xor_any(double x) {
+    if constexpr (XL == 1u)
+        return lane_xor_d<LX_Q1>(x, 1);
+    else if constexpr (XL == 2u)
+        return lane_xor_d<LX_Q2>(x, 2);
+    else if constexpr (XL == 8u)
+        return lane_xor_d<LX_R8>(x, 8);
+    else if constexpr (XL == 4u)
+        return lane_xor_d<LX_R4>(x, 4);
+    else
+        return __shfl_xor(x, (int)XL, 64);
+}
+
+// w[r] = conj(a_{i^x}) a_i for the thread's slots (XT: x as tile positions;
+// WAVE: the tile is in LDS, lds[tid | r << TB])
+template <uint32_t XT>
+__device__ __forceinline__ void sweep_w(dv2 (&w)[TR], const dv2 (&v)[TR], const dv2 *lds, uint32_t tid) {
+    constexpr uint32_t XR = XT >> TB, XTH = XT & (uint32_t)(TT - 1), XL = XT & 63u;
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const dv2 m = v[r];
+        if constexpr (XT == 0) {
+            w[r] = mk2(m.x * m.x + m.y * m.y, 0.0);
+        } else {
+            dv2 o;
+            if constexpr (XTH != XL)
+                o = lds[(tid ^ XTH) | ((uint32_t)(r ^ (int)XR) << TB)];
+            else if constexpr (XL != 0)
+                o = mk2(lane_xor_any<XL>(v[r ^ (int)XR].x), lane_xor_any<XL>(v[r ^ (int)XR].y));
+            else
+                o = v[r ^ (int)XR];
+            w[r] = mk2(o.x * m.x + o.y * m.y, o.x * m.y - o.y * m.x);
+        }
+    }
+}
+
+// sum over the thread's slots of (-1)^popcount(r & ZR) Re(i^Q w[r])
+// (Re(i^q w) = w.x, -w.y, -w.x, w.y for q = 0..3)
+template <uint32_t ZR, uint32_t Q>
+__device__ __forceinline__ double sweep_sum(const dv2 (&w)[TR]) {
+    double sp = 0.0, sn = 0.0;
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        const double t = (Q & 1u) ? w[r].y : w[r].x;
+        const bool neg = ((__builtin_popcount((uint32_t)r & ZR) & 1) != 0) != (((Q + 1u) & 2u) != 0);
+        if (neg)
+            sn += t;
+        else
+            sp += t;
+    }
+    return sp - sn;
+}
+
+// the workgroup's sum of v (every thread returns it)
+__device__ __forceinline__ double sweep_block_sum(double v, double *ws) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < TT / 64; ++w) t += ws[w];
+    __syncthreads();
+    return t;
+}
+
 }  // namespace
 }  // namespace tile
 }  // namespace kbwh

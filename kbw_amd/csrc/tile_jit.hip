@@ -248,6 +248,71 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
     return s;
 }
 
+// The kernel of one exp_value sweep (11-qubit tiles; see sweep_w in
+// tile_dev.h): every workgroup walks tiles blockIdx.x + j gridDim.x, reads
+// each tile once, forms w per distinct x of the group's strings and adds each
+// string's weighted sum; one partial sum per workgroup.
+std::string sweep_source(const SweepKey *k, uint32_t count, const std::string &name) {
+    bool wave = false;
+    for (uint32_t i = 0; i < count; ++i) wave = wave || (k[i].xt & 0xC0u) != 0;
+    std::string s;
+    s.reserve(2048 + count * 160);
+    s += "#include \"tile_dev.h\"\nusing namespace kbwh::tile;\n";
+    s += "extern \"C\" __global__ __launch_bounds__(TT) void " + name +
+         "(const dv2 *__restrict__ a, TileSpec ts, const SweepString *__restrict__ ss, uint32_t ntiles,\n"
+         "    double *__restrict__ partial) {\n";
+    s += wave ? "  __shared__ dv2 lds[TT * TR];\n" : "  const dv2 *lds = nullptr;\n";
+    s += "  __shared__ double ws[TT / 64];\n"
+         "  const uint32_t tid = threadIdx.x;\n"
+         "  uint64_t tmask = 0;\n"
+         "#pragma unroll\n"
+         "  for (int b = 0; b < TB; ++b)\n"
+         "    if ((tid >> b) & 1u) tmask |= ts.qbit[b];\n"
+         "  double acc = 0.0;\n"
+         "#pragma nounroll\n"
+         "  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {\n"
+         "    const uint64_t dt = tile_base(t, ts) | tmask;\n"
+         "    dv2 v[TR];\n"
+         "#pragma unroll\n"
+         "    for (int r = 0; r < TR; ++r) v[r] = __builtin_nontemporal_load(a + slot_offset(dt, r, ts));\n";
+    if (wave)
+        s += "    __syncthreads();  // the previous tile's partner reads are done\n"
+             "#pragma unroll\n"
+             "    for (int r = 0; r < TR; ++r) lds[tid | (r << TB)] = v[r];\n"
+             "    __syncthreads();\n";
+    s += "    dv2 w[TR];\n";
+    // strings with the same x share w; groups in order of first appearance
+    std::vector<bool> done(count, false);
+    char line[256];
+    for (uint32_t i = 0; i < count; ++i) {
+        if (done[i]) continue;
+        snprintf(line, sizeof line, "    sweep_w<%uu>(w, v, lds, tid);\n", k[i].xt);
+        s += line;
+        for (uint32_t j = i; j < count; ++j) {
+            if (done[j] || k[j].xt != k[i].xt) continue;
+            done[j] = true;
+            snprintf(line, sizeof line,
+                     "    { const SweepString &S = ss[%u]; const double sk = sweep_sum<%uu, %uu>(w);\n"
+                     "      acc += ((__popcll(dt & S.z) & 1) ? -S.coeff : S.coeff) * sk; }\n",
+                     j, k[j].zr, k[j].q & 3u);
+            s += line;
+        }
+    }
+    s += "  }\n"
+         "  const double total = sweep_block_sum(acc, ws);\n"
+         "  if (tid == 0) partial[blockIdx.x] = total;\n"
+         "}\n";
+    return s;
+}
+
+std::string sweep_key(const SweepKey *k, uint32_t count) {
+    std::string key;
+    const uint32_t tag = kJitSweep;
+    key.append((const char *)&tag, sizeof tag);
+    for (uint32_t i = 0; i < count; ++i) key.append((const char *)&k[i], 3 * sizeof(uint32_t));
+    return key;
+}
+
 // The reverse sweep's record (adj_dev.h's AGate; TGate does not depend on
 // the tile width).
 struct AGateRec {
@@ -421,6 +486,8 @@ bool compile_source(const std::string &src, std::vector<char> &code, std::string
 // attempt's code even if it spills (inspection).
 bool compile_pass(const JitPass &p, const std::string &name, std::vector<char> &code, std::string &log,
                   bool allow_scratch) {
+    if (p.flags & kJitSweep)
+        return compile_source(sweep_source((const SweepKey *)p.gates, p.count, name), code, log, allow_scratch);
     if (p.flags & kJitAdjoint)
         return compile_source(adj_pass_source((const AGateRec *)p.gates, p.count, name), code, log, allow_scratch);
     const TGate *g = (const TGate *)p.gates;
@@ -485,7 +552,8 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
             const bool adj = (fl & kJitAdjoint) != 0;
             const TGate *g = (const TGate *)passes[i].gates;
             const AGateRec *ag = (const AGateRec *)passes[i].gates;
-            std::string key = adj ? adj_pass_key(ag, cnt) : pass_key(g, cnt, fl);
+            std::string key = (fl & kJitSweep) ? sweep_key((const SweepKey *)passes[i].gates, cnt)
+                              : adj ? adj_pass_key(ag, cnt) : pass_key(g, cnt, fl);
             auto it = C.kernels.find(key);
             if (it != C.kernels.end()) {
                 ks[i] = it->second;
@@ -646,6 +714,16 @@ void order_report(OrderTrial &t) {
     for (uint32_t i = 1; i < kNumOrders; ++i)
         if (o.ms[i] >= 0.0f && (o.ms[b] < 0.0f || o.ms[i] < o.ms[b])) b = i;
     o.best = kOrders[b];
+}
+
+hipError_t jit_launch_sweep(void *fn, const double2 *amp, const void *spec, const void *strings, uint32_t ntiles,
+                            double *partial, uint32_t grid, hipStream_t stream) {
+    TileSpec ts;
+    memcpy(&ts, spec, sizeof ts);
+    const dv2 *a = (const dv2 *)amp;
+    const void *ss = strings;
+    void *args[] = {&a, &ts, &ss, &ntiles, &partial};
+    return hipModuleLaunchKernel((hipFunction_t)fn, grid, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
 }
 
 hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates,

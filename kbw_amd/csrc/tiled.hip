@@ -285,11 +285,39 @@ int32_t exp_value_tiled(kbwh_state *s, const std::vector<PauliMasks> &ms, const 
     }
     const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
     const uint32_t grid = std::min<uint32_t>(ntiles, 2048);
+    // with kbwh_set_compiled, each group's sweep runs a kernel compiled for
+    // its strings (tile_jit.hip); a work-area call (no device allocation)
+    // keeps the generic kernel, since loading a module allocates
+    std::vector<void *> fns;
+    if (s->compiled_passes && !work && e == hipSuccess) {
+        std::vector<std::vector<SweepKey>> keys(groups.size());
+        std::vector<JitPass> lst;
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            const SGroup &g = groups[gi];
+            for (uint32_t k = 0; k < g.count; ++k) {
+                const SString &S = strings[g.first + k];
+                SweepKey key{S.xt, 0u, S.ny & 3u, 0u};
+                for (int b = 0; b < TL - TB; ++b)
+                    if (g.ts.qbit[TB + b] & S.z) key.zr |= 1u << b;
+                keys[gi].push_back(key);
+            }
+            lst.push_back(JitPass{keys[gi].data(), g.count, kJitSweep});
+        }
+        rc = jit_prepare(s->device, lst, fns);
+        if (rc) {
+            (void)hipStreamSynchronize(s->stream);  // the string upload reads the host table
+            return rc;
+        }
+    }
     for (size_t gi = 0; gi < groups.size() && e == hipSuccess; ++gi) {
         const SGroup &g = groups[gi];
-        k_pauli_tile<<<grid, TT, 0, s->stream>>>((const dv2 *)s->amp, g.ts, ds + g.first, g.count, g.has_x, ntiles,
-                                                  partial);
-        e = hipGetLastError();
+        if (gi < fns.size() && fns[gi]) {
+            e = jit_launch_sweep(fns[gi], s->amp, &g.ts, ds + g.first, ntiles, partial, grid, s->stream);
+        } else {
+            k_pauli_tile<<<grid, TT, 0, s->stream>>>((const dv2 *)s->amp, g.ts, ds + g.first, g.count, g.has_x,
+                                                      ntiles, partial);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess) e = launch_sum_final(partial, grid, results + gi, s->stream);
     }
     if (e != hipSuccess) {

@@ -203,7 +203,8 @@ class DenseHip:
         check(self._lib.kbwh_run(self._h, ptr(a), len(a), ptr(p), mode))
 
     def set_compiled(self, enable=True):
-        """Adjoint gradients on kernels compiled per pass (kbwh_set_compiled)."""
+        """Adjoint gradients on kernels compiled per pass, and exp_value's
+        grouped sweeps on kernels compiled per string group (kbwh_set_compiled)."""
         check(self._lib.kbwh_set_compiled(self._h, 1 if enable else 0))
 
     def set_plan_search(self, trials):

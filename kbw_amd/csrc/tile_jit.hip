@@ -109,7 +109,8 @@ uint64_t fnv1a(const std::string &s) {
 // (JIT_INIT), the amplitude mask of one set: the tile starts as |0...0> of
 // its set in registers instead of being loaded.
 // A wide pass (JIT_WIDE) runs 12-qubit tiles, 16 amplitudes per thread,
-// built for `waves` waves per SIMD (the register budget: 3 gives 168 VGPRs).
+// built for `waves` waves per SIMD (the register budget: 4 gives 128 VGPRs,
+// 3 gives 168).
 std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const std::string &name, int waves = 3) {
     std::string s;
     s.reserve(1024 + count * 200);
@@ -232,8 +233,8 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
         snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tg, G); }\n", G.kind,
                  tp, cm, dlo, dhi);
         s += line;
-        // one gate at a time: scheduled across gates, long passes kept up to
-        // 198 registers live (2 waves per SIMD instead of 3)
+        // one gate at a time: scheduled across gates, the temporaries of
+        // several gates are live at once
         s += "  __builtin_amdgcn_sched_barrier(0);\n";
     }
     if (any_phase) s += "  tile_phase_apply(v, ph);\n";
@@ -481,9 +482,9 @@ bool compile_source(const std::string &src, std::vector<char> &code, std::string
     return true;
 }
 
-// The kernel of one pass under `name`.  A wide pass that spills at 3 waves
-// per SIMD is built again for 2 (256 VGPRs); allow_scratch keeps the last
-// attempt's code even if it spills (inspection).
+// The kernel of one pass under `name`.  A wide pass is built for 4 waves per
+// SIMD (128 VGPRs), again for 3 (168) if that spills, then for 2 (256);
+// allow_scratch keeps the last attempt's code even if it spills (inspection).
 bool compile_pass(const JitPass &p, const std::string &name, std::vector<char> &code, std::string &log,
                   bool allow_scratch) {
     if (p.flags & kJitSweep)

@@ -132,6 +132,7 @@ constexpr uint32_t kJitInit = 1;     // the pass starts from |0...0> of each set
 constexpr uint32_t kJitAdjoint = 2;  // a reverse-sweep pass of the adjoint (records: the sweep's AGate)
 constexpr uint32_t kJitWide = 4;     // 12-qubit tiles (records and TileSpec of the wide planner)
 constexpr uint32_t kJitSweep = 8;    // an exp_value sweep of one string group (records: SweepKey)
+constexpr uint32_t kJitPermStore = 16;  // Block swaps folded in: the tile is stored through a second TileSpec
 // the compile-time part of one string of a sweep: x as tile positions, z on
 // the register positions, ny mod 4
 struct SweepKey {
@@ -147,8 +148,10 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
 int32_t jit_code(const JitPass &p, std::vector<char> &code);
 // set_shift < 32: per-set records (tile t takes gates + (t >> set_shift) * count);
 // set_mask: the amplitude mask of one set (kJitInit passes)
+// store_spec: the TileSpec the tile is stored through (kJitPermStore kernels)
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
-                      hipStream_t stream, uint32_t set_shift = 32, uint64_t set_mask = ~0ull);
+                      hipStream_t stream, uint32_t set_shift = 32, uint64_t set_mask = ~0ull,
+                      const void *store_spec = nullptr);
 // an exp_value sweep (11-qubit tiles): strings = the group's device string
 // records; one partial sum per workgroup into partial[0..grid)
 hipError_t jit_launch_sweep(void *fn, const double2 *amp, const void *spec, const void *strings, uint32_t ntiles,

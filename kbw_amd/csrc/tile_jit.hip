@@ -122,7 +122,9 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
          "KBWH_TILE_WAVES))) void ";
     s += name;
     s += "(dv2 *__restrict__ a, TileSpec ts, const TGate *__restrict__ gp, uint32_t ntiles, uint32_t set_shift,\n"
-         "    uint64_t set_mask, uint32_t order) {\n"
+         "    uint64_t set_mask, uint32_t order";
+    s += (flags & kJitPermStore) ? ", TileSpec tss) {\n" : ") {\n";
+    s += 
          "  __shared__ dv2 lds[TT * kChunk];\n"
          "  const uint32_t tid = threadIdx.x;\n"
          "  const uint32_t tile = tile_of_block(blockIdx.x, ntiles, order);\n"
@@ -238,7 +240,14 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
         s += "  __builtin_amdgcn_sched_barrier(0);\n";
     }
     if (any_phase) s += "  tile_phase_apply(v, ph);\n";
-    if (moved) {
+    if (flags & kJitPermStore) {
+        // Block swaps folded into the pass: the tile goes through tss to
+        // addresses that other waves of the workgroup loaded, so every load
+        // must have returned before any store is issued
+        s += "  __builtin_amdgcn_s_waitcnt(0);\n"
+             "  __syncthreads();\n"
+             "  tile_store(v, a, thread_offset(base, tid, tss), tss);\n}\n";
+    } else if (moved) {
         // the store through the final layout: position p holds the qubit loaded at at[p]
         s += "  tile_store_at<";
         for (int q = 0; q < tl; ++q) s += std::to_string(at[q]) + (q + 1 < tl ? ", " : "");
@@ -613,13 +622,14 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
 }
 
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
-                      hipStream_t stream, uint32_t set_shift, uint64_t set_mask) {
-    TileSpec ts;
+                      hipStream_t stream, uint32_t set_shift, uint64_t set_mask, const void *store_spec) {
+    TileSpec ts, tss;
     memcpy(&ts, spec, sizeof ts);
+    if (store_spec) memcpy(&tss, store_spec, sizeof tss);
     dv2 *a = (dv2 *)amp;
     const TGate *g = (const TGate *)gates;
     uint32_t order = kDefaultTileOrder;
-    void *args[] = {&a, &ts, &g, &ntiles, &set_shift, &set_mask, &order};
+    void *args[] = {&a, &ts, &g, &ntiles, &set_shift, &set_mask, &order, &tss};
     return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
 }
 

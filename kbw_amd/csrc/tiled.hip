@@ -482,7 +482,7 @@ struct PassBuilder {
             std::vector<JitPass> lst;
             lst.reserve(launches.size());
             for (const Launch &L : launches)
-                lst.push_back(JitPass{L.perm ? nullptr : (const void *)&gates[L.first], L.perm ? 0u : L.count, 0u});
+                lst.push_back(JitPass{&gates[L.first], L.count, L.perm ? kJitPermStore : 0u});
             const int32_t rc = jit_prepare(s->device, lst, fns);
             if (rc) return rc;
         }
@@ -494,7 +494,8 @@ struct PassBuilder {
         for (size_t i = 0; i < launches.size(); ++i) {
             const Launch &L = launches[i];
             if (!fns.empty() && fns[i])
-                e = jit_launch(fns[i], amp, &L.spec, dg + L.first, ntiles, s->stream);
+                e = jit_launch(fns[i], amp, &L.spec, dg + L.first, ntiles, s->stream, 32, ~0ull,
+                               L.perm ? &L.store : nullptr);
             else if (contract && !L.perm)
                 e = launch_tile_contracted(amp, &L.spec, dg + L.first, L.count, ntiles, s->stream);
             else
@@ -725,7 +726,7 @@ int32_t run_tiled_on(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t n
         if (rc || done) return rc;
     }
     PassBuilder pb(s, amp, reorder && !s->block);
-    pb.compiled = compiled && reorder && !s->block;
+    pb.compiled = compiled && reorder;
     // the map replayed in program order, exactly as gate by gate: a Block
     // state's swaps become events before the gates that need them
     auto to_events = [&](std::vector<kbwh_phys_event> &ev) {

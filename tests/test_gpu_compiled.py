@@ -251,3 +251,38 @@ def test_compiled_tile_order_tuning_keeps_the_bits():
     o.run(arr)
     assert float(np.max(np.abs(states[0] - o.state()))) <= 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("n,L,kind", [(14, 12, "random"), (16, 12, "qft"), (23, 18, "qft"), (20, 16, "random")])
+def test_compiled_block_states(n, L, kind):
+    """A Block state in KBWH_RUN_COMPILED: the reference's map and swap count
+    (decided in program order), passes with folded global/local swaps run on
+    kernels compiled with a second store spec (the tile written through the
+    exchanged positions after every load has returned), the physical
+    amplitudes against the oracle's Block to 1e-12 and against
+    KBWH_RUN_REORDER's generic passes to rounding; no kernel fails to
+    compile."""
+    print(f"compiled block n={n} L={L} {kind}", flush=True)
+    if kind == "qft":
+        ops = circuits.qft_plus_brickwork(n, 4, seed=2)
+    else:
+        ops = circuits.random_circuit(np.random.default_rng(31 + n), n, 300, max_controls=2)
+    before = compiled_stats()
+    g = DenseHip(n, block=True, block_size=L)
+    g.run(ops, mode=RUN_COMPILED)
+    after = compiled_stats()
+    assert after["failures"] == before["failures"]
+    assert after["compiled"] + after["cache_hits"] > before["compiled"] + before["cache_hits"]
+    r = DenseHip(n, block=True, block_size=L)
+    r.run(ops, mode=RUN_REORDER)
+    o = oracle.Block(n, L)
+    for op in ops:
+        m = int(op["ctrl_mask"])
+        o.gate(int(op["kind"]), float(op["angle"]), int(op["target"]), [q for q in range(n) if m >> q & 1])
+    assert np.array_equal(g.qubit_map(), o.qubit_map())
+    assert g.swap_count() == o.swap_count() > 0
+    st = g.state()
+    assert np.max(np.abs(st - o.physical_state())) < 1e-12
+    assert np.max(np.abs(st - r.state())) < 1e-13
+    g.close()
+    r.close()

@@ -151,7 +151,7 @@ int32_t jit_code(const JitPass &p, std::vector<char> &code);
 // store_spec: the TileSpec the tile is stored through (kJitPermStore kernels)
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
                       hipStream_t stream, uint32_t set_shift = 32, uint64_t set_mask = ~0ull,
-                      const void *store_spec = nullptr);
+                      const void *store_spec = nullptr, uint32_t order = 3);
 // an exp_value sweep (11-qubit tiles): strings = the group's device string
 // records; one partial sum per workgroup into partial[0..grid)
 hipError_t jit_launch_sweep(void *fn, const double2 *amp, const void *spec, const void *strings, uint32_t ntiles,

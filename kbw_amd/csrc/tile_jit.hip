@@ -622,13 +622,13 @@ int32_t jit_prepare(int device, const std::vector<JitPass> &passes, std::vector<
 }
 
 hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gates, uint32_t ntiles,
-                      hipStream_t stream, uint32_t set_shift, uint64_t set_mask, const void *store_spec) {
+                      hipStream_t stream, uint32_t set_shift, uint64_t set_mask, const void *store_spec,
+                      uint32_t order) {
     TileSpec ts, tss;
     memcpy(&ts, spec, sizeof ts);
     if (store_spec) memcpy(&tss, store_spec, sizeof tss);
     dv2 *a = (dv2 *)amp;
     const TGate *g = (const TGate *)gates;
-    uint32_t order = kDefaultTileOrder;
     void *args[] = {&a, &ts, &g, &ntiles, &set_shift, &set_mask, &order, &tss};
     return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
 }
@@ -648,6 +648,8 @@ constexpr uint32_t kOrders[] = {3, 5, 9, 0};
 constexpr uint32_t kNumOrders = (uint32_t)(sizeof kOrders / sizeof kOrders[0]);
 // smaller states are not tuned (fewer tiles than a few waves of workgroups)
 constexpr uint32_t kMinTunedTiles = 1u << 12;
+// layouts kept for the process; beyond this, new layouts run the default order
+constexpr size_t kMaxTunedLayouts = 16384;
 
 struct OrderSlot {
     float ms[kNumOrders];
@@ -682,7 +684,12 @@ OrderTrial order_pick(int device, void *fn, const void *spec, size_t spec_bytes,
     key.append((const char *)&ntiles, sizeof ntiles);
     key.append((const char *)spec, spec_bytes);
     std::lock_guard<std::mutex> lk(T.mu);
-    OrderSlot &o = T.slots[key];
+    auto it = T.slots.find(key);
+    if (it == T.slots.end()) {
+        if (T.slots.size() >= kMaxTunedLayouts) return t;
+        it = T.slots.emplace(key, OrderSlot()).first;
+    }
+    OrderSlot &o = it->second;
     if (!T.enabled.load() || o.next >= kNumOrders) {
         t.order = T.enabled.load() ? o.best : kDefaultTileOrder;
         return t;

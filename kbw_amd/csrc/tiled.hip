@@ -491,18 +491,48 @@ struct PassBuilder {
         s->last_passes += launches.size();
         const uint32_t ntiles = (uint32_t)(1ull << (n - TL));
         const TGate *dg = (const TGate *)s->tile_gates;
-        for (size_t i = 0; i < launches.size(); ++i) {
+        // compiled passes tune their tile order on their first runs (order_pick)
+        std::vector<OrderTrial> trials;
+        bool timing = false;
+        for (size_t i = 0; i < launches.size() && e == hipSuccess; ++i) {
             const Launch &L = launches[i];
-            if (!fns.empty() && fns[i])
-                e = jit_launch(fns[i], amp, &L.spec, dg + L.first, ntiles, s->stream, 32, ~0ull,
-                               L.perm ? &L.store : nullptr);
-            else if (contract && !L.perm)
+            if (!fns.empty() && fns[i]) {
+                unsigned char key[2 * sizeof(TileSpec)];
+                size_t kb = sizeof(TileSpec);
+                memcpy(key, &L.spec, kb);
+                if (L.perm) {
+                    memcpy(key + kb, &L.store, sizeof(TileSpec));
+                    kb += sizeof(TileSpec);
+                }
+                trials.push_back(order_pick(s->device, fns[i], key, kb, ntiles));
+                OrderTrial &t = trials.back();
+                timing = timing || t.slot;
+                if (t.slot) e = hipEventRecord(t.ev[0], s->stream);
+                if (e == hipSuccess)
+                    e = jit_launch(fns[i], amp, &L.spec, dg + L.first, ntiles, s->stream, 32, ~0ull,
+                                   L.perm ? &L.store : nullptr, t.order);
+                if (e == hipSuccess && t.slot) e = hipEventRecord(t.ev[1], s->stream);
+            } else if (contract && !L.perm) {
                 e = launch_tile_contracted(amp, &L.spec, dg + L.first, L.count, ntiles, s->stream);
-            else
+            } else {
                 e = launch_tile_exact(amp, &L.spec, L.perm ? &L.store : nullptr, dg + L.first, L.count, ntiles,
                                       s->stream);
-            if (e != hipSuccess) return hip_status(e, "k_tile launch");
+            }
         }
+        if (timing) {
+            // a run with order trials waits for them and reports their times
+            const hipError_t se = hipStreamSynchronize(s->stream);
+            for (OrderTrial &t : trials) {
+                if (e == hipSuccess && se == hipSuccess) {
+                    order_report(t);
+                } else if (t.slot) {
+                    (void)hipEventDestroy(t.ev[0]);
+                    (void)hipEventDestroy(t.ev[1]);
+                }
+            }
+            if (e == hipSuccess) e = se;
+        }
+        if (e != hipSuccess) return hip_status(e, "k_tile launch");
         // the gate buffer is reused by the next run: the upload above is
         // stream-ordered, and the next upload waits for these kernels
         launches.clear();

@@ -581,10 +581,11 @@ def cpu_baseline(n, ops, ngates):
 
 def pergate_table_leg(sim, stream, n, reps=3):
     """The plain per-gate path (one kbwh_gate_mask launch over the state) for
-    X, H, RY, RZ, P and CZ on every target: median launch time over `reps`
-    launches (HIP events on the engine's stream) and the fraction of 8 TB/s
-    achieved on the gate's algorithmic bytes (read + write of each touched
-    amplitude: 32 B x 2^n, 2^(n-1) for P, 2^(n-2) for CZ).  Z and P below
+    X, H, RY, RZ, P, CZ and CNOT on every target (CZ's and CNOT's control is
+    the target's neighbour): median launch time over `reps` launches (HIP
+    events on the engine's stream) and the fraction of 8 TB/s achieved on the
+    gate's algorithmic bytes (read + write of each touched amplitude: 32 B x
+    2^n, 2^(n-1) for P and CNOT, 2^(n-2) for CZ).  Z and P below
     bit 3 (and CZ there) touch part of every 128-byte line, so whole lines
     move and their fraction is capped near 0.5 (0.25 for CZ) of the
     streaming rate."""
@@ -595,12 +596,12 @@ def pergate_table_leg(sim, stream, n, reps=3):
 
     lib, h = sim._lib, sim.handle
     kinds = {"X": (_lib.X, 0.0), "H": (_lib.H, 0.0), "RY": (_lib.RY, 0.7), "RZ": (_lib.RZ, 0.7),
-             "P": (_lib.P, 0.7), "CZ": (_lib.Z, 0.0)}
+             "P": (_lib.P, 0.7), "CZ": (_lib.Z, 0.0), "CNOT": (_lib.X, 0.0)}
     table = {}
     for name, (kind, ang) in kinds.items():
         row = []
         for t in range(n):
-            mask = (1 << (t + 1 if t + 1 < n else t - 1)) if name == "CZ" else 0
+            mask = (1 << (t + 1 if t + 1 < n else t - 1)) if name in ("CZ", "CNOT") else 0
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
             ev[0].record(stream)
             for r in range(reps):
@@ -614,7 +615,7 @@ def pergate_table_leg(sim, stream, n, reps=3):
     summary = {k: {"mean_frac": float(np.mean(v)), "min_frac": float(np.min(v)),
                    "min_target": int(np.argmin(v))} for k, v in table.items()}
     return {"frac_by_kind_and_target": table, "summary": summary, "reps": reps,
-            "note": "fraction of 8 TB/s on algorithmic bytes per launch; CZ's control is the target's neighbour"}
+            "note": "fraction of 8 TB/s on algorithmic bytes per launch; CZ's and CNOT's control is the target's neighbour"}
 
 
 def reorder_leg(sim, ops, n, args, per_launch, alg_bytes, tol=1e-12, compiled=False):

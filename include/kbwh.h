@@ -250,7 +250,9 @@ int32_t kbwh_exp_value(kbwh_state *s, const uint8_t *paulis, const uint32_t *qub
  * (KBWH_ERR_INVALID_ARGUMENT: call kbwh_synchronize first, since a flush may
  * allocate).  All calls on one GPU share the array: calls that overlap in
  * time on one GPU (two threads, two handles) race and must be serialised by
- * the caller.  The result bits equal kbwh_exp_value's. */
+ * the caller.  The result bits equal kbwh_exp_value's on a handle without
+ * kbwh_set_compiled (with it, kbwh_exp_value's compiled sweeps agree to
+ * rounding, this variant keeping the generic sweep). */
 int32_t kbwh_exp_value_noalloc(kbwh_state *s, const uint8_t *paulis, const uint32_t *qubits,
                                const uint32_t *offsets, const double *coeffs, uint32_t num_strings,
                                double *out);

@@ -132,18 +132,19 @@ __global__ __launch_bounds__(kBlock) void k_diag(double2 *__restrict__ a, uint64
 
 // Low targets (pair gates below bit 6, Z and P below bit 3; also RZ over
 // more than 2^31 amplitudes, which this kernel's grid stride covers): each lane
-// takes one amplitude, so every wave reads and writes 64 consecutive
-// amplitudes (1 KiB), and gets the other amplitude of its pair from the lane
-// across the target bit (a cross-lane shuffle).  The pair is computed with
-// the reference's arithmetic, and each lane keeps its own half, so the
-// results are the same bits as k_pair / k_diag.  Controls at bit 6 and above
-// are inserted into the index (only control-satisfied 64-amplitude segments
-// are visited); controls below bit 6 are tested per lane (lo_ctrl).  Z and P
-// on bits 0..2 touch every other amplitude of each 128-byte line, so the
-// whole line is read and written back either way; streaming whole lines
-// avoids partial-line writes.
+// takes one amplitude, so every wave reads and writes 64 amplitudes of the
+// control-satisfied subspace, and gets the other amplitude of its pair from
+// the lane across the target's bit of the compressed index (lm; a cross-lane
+// shuffle).  The pair is computed with the reference's arithmetic, and each
+// lane keeps its own half, so the results are the same bits as k_pair /
+// k_diag.  Controls at bit 3 and above are inserted into the index (only
+// control-satisfied runs of at least 8 amplitudes, whole 128-byte lines, are
+// visited); controls below bit 3 are tested per lane (lo_ctrl).  Z and P on
+// bits 0..2 touch every other amplitude of each 128-byte line, so the whole
+// line is read and written back either way; streaming whole lines avoids
+// partial-line writes.
 template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_low(double2 *__restrict__ a, uint64_t count, uint32_t t,
+__global__ __launch_bounds__(kBlock) void k_low(double2 *__restrict__ a, uint64_t count, uint32_t t, uint32_t lm,
                                                 uint64_t lo_ctrl, uint64_t set_hi, Insert ins_hi, double c,
                                                 double s, double2 ph0, double2 ph1) {
     // count and the stride are multiples of 64, so a wave is either wholly
@@ -161,8 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_low(double2 *__restrict__ a, uint64_
         } else {
             // component-wise selects keep the pair in registers (a select of
             // whole double2 objects is lowered to a select of addresses)
-            const int lm = 1 << t;
-            const double ox = __shfl_xor(v.x, lm, 64), oy = __shfl_xor(v.y, lm, 64);
+            const double ox = __shfl_xor(v.x, (int)lm, 64), oy = __shfl_xor(v.y, (int)lm, 64);
             double2 a0 = make_double2(one ? ox : v.x, one ? oy : v.y);
             double2 a1 = make_double2(one ? v.x : ox, one ? v.y : oy);
             pair_math<KIND>(a0, a1, c, s);
@@ -491,24 +491,27 @@ hipError_t launch_gate(double2 *a, uint32_t n, int kind, double angle, uint32_t 
         return hipGetLastError();
     }
     // low targets: one amplitude per lane, the pair's partner from another
-    // lane (needs whole 64-amplitude segments: the controls at bit 6 and up
-    // leave at least 6 free bits)
-    const uint64_t hi_ctrl = ctrl_mask & ~63ull, lo_ctrl = ctrl_mask & 63ull;
+    // lane (needs whole 64-amplitude segments of the compressed index: the
+    // controls inserted at bit 3 and up leave at least 6 free bits)
+    const uint64_t hi_ctrl = ctrl_mask & ~7ull, lo_ctrl = ctrl_mask & 7ull;
     const uint32_t c_hi = (uint32_t)__builtin_popcountll(hi_ctrl);
     const bool diag = kind == KZ || kind == KP;
     if ((kind == KRZ || target < (diag ? 3u : 6u)) && n >= 6 + c_hi) {
         const uint64_t cnt = 1ull << (n - c_hi);
         const Insert ins = make_insert(hi_ctrl);
+        // the target's bit in the compressed index (inserted controls below it removed)
+        const uint32_t tc = target - (uint32_t)__builtin_popcountll(hi_ctrl & (tbit - 1));
+        const uint32_t lm = target < 6u ? 1u << tc : 0u;
         const uint32_t g = (uint32_t)std::min<uint64_t>(grid_for(cnt, kBlock), 1u << 22);  // 2^30 lanes at most
         switch (kind) {
-        case KX: k_low<KX><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
-        case KY: k_low<KY><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
-        case KZ: k_low<KZ><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
-        case KH: k_low<KH><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
-        case KP: k_low<KP><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
-        case KRX: k_low<KRX><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
-        case KRY: k_low<KRY><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
-        case KRZ: k_low<KRZ><<<g, kBlock, 0, st>>>(a, cnt, target, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KX: k_low<KX><<<g, kBlock, 0, st>>>(a, cnt, target, lm, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KY: k_low<KY><<<g, kBlock, 0, st>>>(a, cnt, target, lm, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KZ: k_low<KZ><<<g, kBlock, 0, st>>>(a, cnt, target, lm, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KH: k_low<KH><<<g, kBlock, 0, st>>>(a, cnt, target, lm, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KP: k_low<KP><<<g, kBlock, 0, st>>>(a, cnt, target, lm, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KRX: k_low<KRX><<<g, kBlock, 0, st>>>(a, cnt, target, lm, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KRY: k_low<KRY><<<g, kBlock, 0, st>>>(a, cnt, target, lm, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
+        case KRZ: k_low<KRZ><<<g, kBlock, 0, st>>>(a, cnt, target, lm, lo_ctrl, hi_ctrl, ins, k.c, k.s, k.ph0, k.ph1); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();

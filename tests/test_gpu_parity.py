@@ -65,9 +65,10 @@ def test_single_gates_every_target_bit_exact(n, kind):
 @pytest.mark.parametrize("n", [6, 7, 8, 12])
 def test_low_target_gates_with_low_and_high_controls(n):
     """Targets 0..5 (one amplitude per lane, partner by cross-lane shuffle)
-    under controls below bit 6 (tested per lane), at bit 6 and above
-    (inserted into the index) and both; n=7 and 8 with every high bit a
-    control leave exactly one 64-amplitude segment per launch."""
+    under controls below bit 3 (tested per lane), at bits 3..5 (inserted into
+    the index below the target or above it, which moves the partner's lane),
+    at bit 6 and above and mixed; n=7 and 8 with every high bit a control
+    leave one or two 64-amplitude segments per launch."""
     print(f"low-target gates n={n}", flush=True)
     rng = np.random.default_rng(500 + n)
     g, o = pair(n, random_state(rng, n))
@@ -75,7 +76,8 @@ def test_low_target_gates_with_low_and_high_controls(n):
     for kind in range(8):
         for t in range(6):
             low = [q for q in range(6) if q != t]
-            sets = [[], [low[0]], low[1:3], high[:1], high, [low[2]] + high[-2:]]
+            mid = [q for q in low if q >= 3]
+            sets = [[], [low[0]], low[1:3], high[:1], high, [low[2]] + high[-2:], mid, mid[:1] + [low[0]] + high[:1]]
             for ctrl in sets:
                 ang = float(rng.uniform(-7, 7))
                 g.gate(kind, ang, t, ctrl)

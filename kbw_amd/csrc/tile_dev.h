@@ -808,6 +808,42 @@ __device__ __forceinline__ void tile_phase_apply(dv2 (&v)[TR], dv2 ph) {
     }
 }
 
+// A diagonal gate whose target and in-tile controls all lie on thread bits
+// or outside the tile (no register bit) multiplies every register slot of a
+// thread by the same factor.  Compiled passes fold such gates into the
+// thread's pending phase tp (one complex product per gate instead of one
+// per slot) and apply tp to the slots only where a later gate or exchange
+// moves amplitudes between threads whose pending phases differ, and before
+// the store.  The factor is diag_at's, outside tests included: Z and P, the
+// |1> phase where the thread bits DLO (target and controls) and the outside
+// bits G.ocm (outside controls, and the target when it is outside) are all
+// 1; RZ, the target's half's phase where the controls hold.
+__device__ __forceinline__ void thread_phase_apply(dv2 (&v)[TR], dv2 tp) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        slot_group(r);
+        v[r] = cmul(v[r], tp);
+    }
+}
+
+template <uint32_t KIND, uint32_t TP, uint32_t DLO, bool OUT>
+__device__ __forceinline__ void thread_phase_acc(dv2 &tp, uint64_t base, uint32_t tid, const TGate &G) {
+    const bool ok = (tid & DLO) == DLO && (!OUT || (base & G.ocm) == G.ocm);
+    if constexpr (KIND == KZ) {
+        tp = flip_sign(tp, ok ? 0x80000000u : 0u);
+    } else if constexpr (KIND == KP) {
+        tp = cmul(tp, ok ? mk2(G.ph1x, G.ph1y) : mk2(1.0, 0.0));
+    } else {
+        static_assert(KIND == KRZ && (TP < (uint32_t)TB || TP == kOutside), "thread-level RZ");
+        bool one;
+        if constexpr (TP == kOutside)
+            one = (base & G.otb) != 0;
+        else
+            one = ((tid >> TP) & 1u) != 0;
+        tp = cmul(tp, ok ? (one ? mk2(G.ph1x, G.ph1y) : mk2(G.ph0x, G.ph0y)) : mk2(1.0, 0.0));
+    }
+}
+
 // A Z with outside bits (controls, or its target) and the rest in the tile:
 // without a branch around it.  A per-tile branch around a gate makes the
 // register tile a join of two versions, and a pass with many of them kept

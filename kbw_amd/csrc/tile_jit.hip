@@ -169,15 +169,36 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
         dlo = d & ((1u << kTb) - 1u);
         dhi = d >> kTb;
     };
-    bool any_phase = false;
-    for (uint32_t i = 0; i < count; ++i) any_phase = any_phase || tile_wide(g[i]);
-    if (any_phase) s += "  dv2 ph = mk2(1.0, 0.0);\n";
+    // The pending phase tp of each thread (thread_phase_acc): diagonal gates
+    // wholly outside the tile (one factor per tile) and diagonal gates on
+    // thread bits and outside bits only are folded into it; tp_dep holds the
+    // thread positions (current layout) its factor depends on.  Before a pair
+    // gate or a layout swap moves amplitudes across a thread bit in tp_dep,
+    // tp is applied to the slots and restarts at 1; what is left is applied
+    // before the store.  tp_thread: a thread-level gate was folded since the
+    // last application (tp may differ between lanes); tp_any: anything was.
+    bool any_diag = false;
+    for (uint32_t i = 0; i < count; ++i) any_diag = any_diag || is_diag(g[i]);
+    if (any_diag) s += "  dv2 tp = mk2(1.0, 0.0);\n";
+    uint32_t tp_dep = 0;
+    bool tp_thread = false, tp_any = false;
+    auto flush_phase = [&]() {
+        s += "  thread_phase_apply(v, tp); tp = mk2(1.0, 0.0);\n  __builtin_amdgcn_sched_barrier(0);\n";
+        tp_dep = 0;
+        tp_thread = tp_any = false;
+    };
     for (uint32_t i = 0; i < count; ++i) {
         const TGate &G = g[i];
         if (tile_wide(G)) {
-            snprintf(line, sizeof line, "  tile_phase_acc<%uu>(ph, base, gp[%u]);\n", G.kind, i);
+            snprintf(line, sizeof line, "  tile_phase_acc<%uu>(tp, base, gp[%u]);\n", G.kind, i);
             s += line;
+            tp_any = true;
             continue;
+        }
+        if (!is_diag(G) && G.tpos != kOutside) {
+            // a pair gate (or its layout swap) on a thread bit the pending phase depends on
+            const int p = cur[G.tpos];
+            if (p < kTb && ((tp_dep >> p) & 1u)) flush_phase();
         }
         if (relayout && !is_diag(G) && G.tpos != kOutside) {
             const int p = cur[G.tpos];
@@ -214,6 +235,18 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
         }
         uint32_t tp, cm, dlo, dhi;
         positions(G, tp, cm, dlo, dhi);
+        if (is_diag(G) && dhi == 0 && (G.kind != KRZ || tp == kOutside || tp < (uint32_t)kTb)) {
+            // one factor per thread: folded into tp
+            snprintf(line, sizeof line,
+                     "  { uint32_t tg = tid; asm volatile(\"\" : \"+v\"(tg)); "
+                     "thread_phase_acc<%uu, %uu, %uu, %s>(tp, base, tg, gp[%u]); }\n",
+                     G.kind, tp, dlo, G.ocm ? "true" : "false", i);
+            s += line;
+            tp_dep |= dlo;
+            if (G.kind == KRZ && tp != kOutside) tp_dep |= 1u << tp;
+            tp_thread = tp_any = true;
+            continue;
+        }
         if (G.kind == KZ && G.ocm) {
             snprintf(line, sizeof line,
                      "  { uint32_t tg = tid; asm volatile(\"\" : \"+v\"(tg)); "
@@ -239,7 +272,10 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
         // several gates are live at once
         s += "  __builtin_amdgcn_sched_barrier(0);\n";
     }
-    if (any_phase) s += "  tile_phase_apply(v, ph);\n";
+    if (tp_thread)
+        s += "  thread_phase_apply(v, tp);\n";
+    else if (tp_any)
+        s += "  tile_phase_apply(v, tp);\n";
     if (flags & kJitPermStore) {
         // Block swaps folded into the pass: the tile goes through tss to
         // addresses that other waves of the workgroup loaded, so every load

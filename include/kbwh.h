@@ -153,13 +153,13 @@ int32_t kbwh_set_plan_search(kbwh_state *s, int32_t trials);
  * the wall time spent compiling.  Any pointer may be NULL. */
 int32_t kbwh_compiled_stats(uint64_t *kernels_compiled, uint64_t *cache_hits, uint64_t *failures,
                             uint64_t *generic_passes, double *compile_seconds);
-/* Tile-order tuning of the 12-qubit compiled passes, for the process (no
+/* Tile-order tuning of the compiled passes, for the process (no
  * counterpart in the reference).  The order in which workgroups take a
  * pass's tiles changes how its loads and stores fall on the HBM channels,
  * not its results (every order is a permutation of independent tiles).  Each
- * pass layout (kernel, tile qubits, tile count) of a state of 24 qubits or
- * more tries 4 candidate orders on its first 4 runs, timed around its
- * launch, and keeps the fastest.  Enabled by default; disabled, every pass
+ * pass layout (kernel, tile qubits, tile count) of 4096 tiles or more (24
+ * qubits in 12-qubit tiles, 23 in 11-qubit tiles) tries 4 candidate orders on
+ * its first 4 runs, timed around its launch, and keeps the fastest.  Enabled by default; disabled, every pass
  * runs the default order. */
 int32_t kbwh_set_order_tuning(int32_t enable);
 /* Pass layouts whose order is chosen, layouts still trying candidates,

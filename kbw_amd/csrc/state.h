@@ -156,10 +156,11 @@ hipError_t jit_launch(void *fn, double2 *amp, const void *spec, const void *gate
 // records; one partial sum per workgroup into partial[0..grid)
 hipError_t jit_launch_sweep(void *fn, const double2 *amp, const void *spec, const void *strings, uint32_t ntiles,
                             double *partial, uint32_t grid, hipStream_t stream);
-// a wide pass (spec: the 12-qubit tile::TileSpec of spec_bytes bytes) with
-// the tile order `order` (tile_of_block in tile_dev.h)
+// a wide pass (spec: the wide tile's tile::TileSpec of spec_bytes bytes)
+// with the tile order `order` (tile_of_block in tile_dev.h), `threads` per
+// workgroup (the wide tile's thread count)
 hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates,
-                           uint32_t ntiles, hipStream_t stream, uint32_t order);
+                           uint32_t ntiles, hipStream_t stream, uint32_t order, uint32_t threads);
 // Tile-order tuner for compiled passes (tile_jit.hip).  Each pass layout
 // (kernel, tile qubits, tile count) tries the candidate orders on its first
 // runs, one per run, timed by events around its launch, and then keeps the
@@ -178,8 +179,11 @@ void order_report(OrderTrial &t);
 // Dependency-ordered runs in 12-qubit tiles with pass-specialised kernels
 // (tiled_wide.hip, KBWH_RUN_COMPILED on flat states of 12 qubits and more).
 // *done is false when the run was left to the 11-qubit passes (a pass kernel
-// did not compile; nothing was launched).
+// did not compile; nothing was launched).  A wide tile is kWideTileQubits
+// qubits over 2^kWideTileTBits threads (6 lane bits, the rest wave bits), 16
+// amplitudes per thread.
 constexpr uint32_t kWideTileQubits = 12;
+constexpr uint32_t kWideTileTBits = 8;
 int32_t run_compiled_wide(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint64_t num_ops, const double *params,
                           bool *done);
 // the gate records (tile::TGate of the wide unit) of pass `pass` of the wide

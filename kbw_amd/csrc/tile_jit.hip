@@ -108,14 +108,15 @@ uint64_t fnv1a(const std::string &s) {
 // the records gp + (t >> set_shift) * count) and, for an initialising pass
 // (JIT_INIT), the amplitude mask of one set: the tile starts as |0...0> of
 // its set in registers instead of being loaded.
-// A wide pass (JIT_WIDE) runs 12-qubit tiles, 16 amplitudes per thread,
-// built for `waves` waves per SIMD (the register budget: 4 gives 128 VGPRs,
-// 3 gives 168).
+// A wide pass (JIT_WIDE) runs the wide tile (kWideTileQubits qubits over
+// 2^kWideTileTBits threads, 16 amplitudes per thread), built for `waves`
+// waves per SIMD (the register budget: 4 gives 128 VGPRs, 3 gives 168).
 std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const std::string &name, int waves = 3) {
     std::string s;
     s.reserve(1024 + count * 200);
     if (flags & kJitWide)
-        s += "#define KBWH_TILE_BITS 12\n#define KBWH_TILE_TBITS 8\n#define KBWH_TILE_WAVES " + std::to_string(waves) +
+        s += "#define KBWH_TILE_BITS " + std::to_string(kWideTileQubits) + "\n#define KBWH_TILE_TBITS " +
+             std::to_string(kWideTileTBits) + "\n#define KBWH_TILE_WAVES " + std::to_string(waves) +
              "\n";
     s += "#define KBWH_TILE_CONTRACT 1\n#include \"tile_dev.h\"\nusing namespace kbwh::tile;\n";
     s += "extern \"C\" __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(KBWH_TILE_WAVES, "
@@ -149,8 +150,8 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
     // holding the qubit loaded at position l; at[p]: the load position of
     // the qubit now at p.  The tile is stored through the final layout.
     const bool relayout = (flags & kJitWide) != 0;
-    constexpr int kTb = 8;
-    const int tl = (flags & kJitWide) ? 12 : 11;
+    const int kTb = (flags & kJitWide) ? (int)kWideTileTBits : TB;
+    const int tl = (flags & kJitWide) ? (int)kWideTileQubits : TL;
     int cur[16], at[16];
     for (int q = 0; q < 16; ++q) cur[q] = at[q] = q;
     bool moved = false;
@@ -781,7 +782,7 @@ hipError_t jit_launch_sweep(void *fn, const double2 *amp, const void *spec, cons
 }
 
 hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec_bytes, const void *gates,
-                           uint32_t ntiles, hipStream_t stream, uint32_t order) {
+                           uint32_t ntiles, hipStream_t stream, uint32_t order, uint32_t threads) {
     alignas(8) unsigned char ts[256];
     if (spec_bytes > sizeof ts) return hipErrorInvalidValue;
     memcpy(ts, spec, spec_bytes);
@@ -790,7 +791,7 @@ hipError_t jit_launch_spec(void *fn, double2 *amp, const void *spec, size_t spec
     uint32_t set_shift = 32;
     uint64_t set_mask = ~0ull;
     void *args[] = {&a, ts, &g, &ntiles, &set_shift, &set_mask, &order};
-    return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, TT, 1, 1, 0, stream, args, nullptr);
+    return hipModuleLaunchKernel((hipFunction_t)fn, ntiles, 1, 1, threads, 1, 1, 0, stream, args, nullptr);
 }
 
 int32_t jit_code(const JitPass &p, std::vector<char> &code) {

@@ -9,9 +9,13 @@
 // on the 30-qubit, 20-layer brickwork circuit 16 passes instead of the
 // 11-qubit tile's 21 (greedy) or 19 (searched).  The register tile is twice as
 // large, which only the specialised kernels carry without spilling (built for
-// 3 waves per SIMD, or 2 where 3 spills); the generic kernel keeps 11-qubit
-// tiles.  When a pass kernel cannot be compiled the run falls back to the
-// 11-qubit passes before anything is launched.
+// 4 waves per SIMD, or 3 or 2 where that spills); the generic kernel keeps
+// 11-qubit tiles.  The shape is kWideTileQubits / kWideTileTBits: 13-qubit
+// tiles of 512 threads plan the same circuit in 14 passes, but their passes
+// carry more gates (compute-bound beyond about 60) and their tile sets more
+// scattered high qubits (memory floors up to 12.9 ms), and the run took 124
+// ms against 118.  When a pass kernel cannot be compiled the run falls back to
+// the 11-qubit passes before anything is launched.
 #include <string.h>
 
 #include <vector>
@@ -31,7 +35,7 @@ namespace {
 
 using namespace tile;
 
-static_assert(TL == (int)kWideTileQubits, "wide tile width");
+static_assert(TL == (int)kWideTileQubits && TB == (int)kWideTileTBits, "wide tile shape");
 
 // Passes longer than this are cut (a kernel's straight line compiles slowly
 // past it); the cut keeps the order, so the pieces are passes of their own.
@@ -138,7 +142,7 @@ int32_t run_compiled_wide(kbwh_state *s, double2 *amp, const kbwh_op *ops, uint6
         if (t.slot) e = hipEventRecord(t.ev[0], s->stream);
         if (e == hipSuccess)
             e = jit_launch_spec(fns[i], amp, &passes[i].spec, sizeof(TileSpec), dg + passes[i].first, ntiles,
-                                s->stream, t.order);
+                                s->stream, t.order, (uint32_t)TT);
         if (e == hipSuccess && t.slot) e = hipEventRecord(t.ev[1], s->stream);
     }
     // the host records are local: the upload must finish before they go away

@@ -99,7 +99,7 @@ def compiled_stats():
 
 
 def set_order_tuning(enable=True):
-    """Tile-order tuning of the 12-qubit compiled passes for the process
+    """Tile-order tuning of the compiled passes for the process
     (kbwh_set_order_tuning): each pass layout tries the candidate orders on
     its first runs and keeps the fastest.  The order does not change results."""
     check(_lib.load().kbwh_set_order_tuning(1 if enable else 0))

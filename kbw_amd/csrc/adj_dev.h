@@ -24,6 +24,41 @@ struct AGate {
 
 __device__ __forceinline__ double im_conj_mul(dv2 a, dv2 b) { return a.x * b.y - a.y * b.x; }
 
+// Per-parameter term accumulators in LDS.  A term is first folded over lane
+// bits 5 and 4 (two permlane row swaps and adds), so lanes 0..15 of each wave hold the
+// wave's sum in 16 parts and a slot takes TT / 4 doubles (512 bytes for 256
+// threads) instead of one per thread: 24 slots fit in 12 KiB instead of 48,
+// and LDS no longer holds the reverse kernels to 2 or 3 workgroups per CU.
+// Each owner lane keeps its own entry (zeroed, added to and read by the same
+// thread, so no barrier is needed between the three).  Called where every
+// lane of the workgroup takes the same path (gate records and tile bits are
+// uniform).
+constexpr int kAccPerSlot = TT / 4;
+__device__ __forceinline__ bool adj_acc_owner(uint32_t tid) { return (tid & 48u) == 0u; }
+__device__ __forceinline__ uint32_t adj_acc_index(uint32_t tid) { return ((tid >> 6) << 4) | (tid & 15u); }
+__device__ __forceinline__ void adj_acc_zero(double *acc, uint32_t nslots, uint32_t tid) {
+    if (adj_acc_owner(tid))
+        for (uint32_t s = 0; s < nslots; ++s) acc[s * kAccPerSlot + adj_acc_index(tid)] = 0.0;
+}
+// t[l] + t[l ^ 32], then the same over lane bit 4, by permlane row swaps of
+// two copies (the lower lane's value is the first addend in both lanes)
+__device__ __forceinline__ double fold_lane_bits_45(double t) {
+    double p = t, q = t;
+    swap_rows<5>(p, q);
+    t = p + q;
+    p = t;
+    q = t;
+    swap_rows<4>(p, q);
+    return p + q;
+}
+__device__ __forceinline__ void adj_acc_add(double *acc, uint32_t slot, uint32_t tid, double term) {
+    term = fold_lane_bits_45(term);
+    if (adj_acc_owner(tid)) acc[slot * kAccPerSlot + adj_acc_index(tid)] += term;
+}
+__device__ __forceinline__ double adj_acc_get(const double *acc, uint32_t slot, uint32_t tid) {
+    return adj_acc_owner(tid) ? acc[slot * kAccPerSlot + adj_acc_index(tid)] : 0.0;
+}
+
 __device__ __forceinline__ double adj_block_sum(double v, double *ws) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);

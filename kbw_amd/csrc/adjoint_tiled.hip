@@ -15,14 +15,17 @@
 // both vectors is one uncontrolled gate of the pass.  For RX and RY the term
 // is Im sum_{C(i)} conj(lambda_i) (G phi)_i, where (G phi)_i is the phi
 // amplitude paired with i across the target (times -+i for Y), fetched like
-// a gate's partner (registers, lane shuffles or LDS).  Each thread adds its
-// terms to its own LDS slot per parameterised gate of the pass; a workgroup
-// reduces its slots at the end and the workgroup sums are added in a fixed
-// order.
+// a gate's partner (registers, lane shuffles or LDS).  Each term is folded
+// over lane bits 4 and 5 and added to the lane's LDS entry for that
+// parameterised gate (16 entries per wave, adj_acc_add in adj_dev.h); a
+// workgroup reduces its entries at the end and the workgroup sums are added
+// in a fixed order.
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -60,12 +63,12 @@ __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec t
                                                  uint32_t ng, uint32_t tps, uint32_t wps, uint32_t nslots,
                                                  double *__restrict__ partial) {
     __shared__ dv2 lds[TT * kChunk];
-    __shared__ double acc[kMaxSlots * TT];
+    __shared__ double acc[kMaxSlots * kAccPerSlot];
     __shared__ double ws[TT / 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t set = blockIdx.x / wps, wk = blockIdx.x % wps;
     gs += (size_t)set * ng;
-    for (uint32_t s = 0; s < nslots; ++s) acc[s * TT + tid] = 0.0;
+    adj_acc_zero(acc, nslots, tid);
     uint64_t tmask = 0;
 #pragma unroll
     for (int b = 0; b < TB; ++b)
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec t
                         term += slot_ctrl_ok(tok, r, G.cm) ? c : 0.0;
                     }
                 }
-                acc[A.slot * TT + tid] += term;
+                adj_acc_add(acc, A.slot, tid, term);
             }
             if (G.kind == KZ || G.kind == KP || G.kind == KRZ)
                 apply_diag(v, base, tid, G);
@@ -123,7 +126,7 @@ __global__ __launch_bounds__(TT) void k_adj_tile(dv2 *__restrict__ a, TileSpec t
         for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, ts)] = v[r];
     }
     for (uint32_t s = 0; s < nslots; ++s) {
-        const double v = adj_block_sum(acc[s * TT + tid], ws);
+        const double v = adj_block_sum(adj_acc_get(acc, s, tid), ws);
         if (tid == 0) partial[(size_t)s * gridDim.x + blockIdx.x] = v;
     }
 }
@@ -289,6 +292,33 @@ int32_t compiled_reverse(kbwh_state *s, const std::vector<RPass> &passes, const 
     return jit_prepare(s->device, lst, fns);
 }
 
+// Workgroups of a reverse kernel resident on the device at once (occupancy
+// per CU times the CU count), at most kAdjTiledGrid.  A reverse kernel walks
+// its tiles with a fixed grid; a grid above what is resident runs its last
+// workgroups in a second, partly idle round (their tiles are uniform work):
+// kernels that fit 3 workgroups per CU (768) took 16 rounds of tiles per
+// pass at a grid of 1024 instead of 11.  Cached per kernel and device; fn
+// null is the generic k_adj_tile.
+uint32_t adj_grid_cap(void *fn, int device) {
+    static std::mutex mu;
+    static std::map<std::pair<void *, int>, uint32_t> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(fn, device);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    hipError_t e = fn ? hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (hipFunction_t)fn, TT, 0)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_tile, TT, 0);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    uint32_t cap = kAdjTiledGrid;
+    if (e == hipSuccess && per_cu > 0 && cus > 0)
+        cap = (uint32_t)std::min<uint64_t>(kAdjTiledGrid, (uint64_t)per_cu * (uint64_t)cus);
+    else
+        (void)hipGetLastError();  // a failed query leaves the default grid
+    cache[key] = cap;
+    return cap;
+}
+
 }  // namespace
 
 // The reverse sweep over phi and lambda (s->adj_phi, lambda right after it)
@@ -329,16 +359,17 @@ int32_t adjoint_reverse_tiled(kbwh_state *s, const kbwh_op *ops, uint64_t num_op
     for (size_t i = 0; i < passes.size(); ++i) {
         const RPass &P = passes[i];
         if (e != hipSuccess) break;
-        if (!fns.empty() && fns[i]) {
-            e = jit_launch_adj(fns[i], s->adj_phi, &P.ts, sizeof P.ts, dg + P.first, ntiles, grid, partial, grid,
-                               s->stream);
+        void *fn = fns.empty() ? nullptr : fns[i];
+        const uint32_t g = std::min(grid, adj_grid_cap(fn, s->device));
+        if (fn) {
+            e = jit_launch_adj(fn, s->adj_phi, &P.ts, sizeof P.ts, dg + P.first, ntiles, g, partial, g, s->stream);
         } else {
-            k_adj_tile<<<grid, TT, 0, s->stream>>>((dv2 *)s->adj_phi, P.ts, dg + P.first, P.count, ntiles, grid,
-                                                   P.nslots, partial);
+            k_adj_tile<<<g, TT, 0, s->stream>>>((dv2 *)s->adj_phi, P.ts, dg + P.first, P.count, ntiles, g, P.nslots,
+                                                partial);
             e = hipGetLastError();
         }
         if (e == hipSuccess && P.nslots) {
-            k_sum_slots<<<dim3(P.nslots, 1), 256, 0, s->stream>>>(partial, grid, grid, 0, slots, P.map);
+            k_sum_slots<<<dim3(P.nslots, 1), 256, 0, s->stream>>>(partial, g, g, 0, slots, P.map);
             e = hipGetLastError();
         }
     }
@@ -406,16 +437,19 @@ int32_t adjoint_reverse_sets(kbwh_state *s, double2 *phi, uint32_t n, uint32_t n
     for (size_t i = 0; i < passes.size(); ++i) {
         const RPass &P = passes[i];
         if (e != hipSuccess) break;
-        if (!fns.empty() && fns[i]) {
-            e = jit_launch_adj(fns[i], phi, &P.ts, sizeof P.ts, dg + (size_t)nsets * P.first, tps, wps, partial,
-                               (uint32_t)grid, st);
+        // the pass's workgroups per set within what its kernel keeps resident
+        void *fn = fns.empty() ? nullptr : fns[i];
+        const uint32_t pw = std::max<uint32_t>(1, std::min<uint32_t>(wps, adj_grid_cap(fn, s->device) / nsets));
+        const uint32_t pg = (uint32_t)((uint64_t)nsets * pw);
+        if (fn) {
+            e = jit_launch_adj(fn, phi, &P.ts, sizeof P.ts, dg + (size_t)nsets * P.first, tps, pw, partial, pg, st);
         } else {
-            k_adj_tile<<<(uint32_t)grid, TT, 0, st>>>((dv2 *)phi, P.ts, dg + (size_t)nsets * P.first, P.count, tps,
-                                                      wps, P.nslots, partial);
+            k_adj_tile<<<pg, TT, 0, st>>>((dv2 *)phi, P.ts, dg + (size_t)nsets * P.first, P.count, tps, pw,
+                                          P.nslots, partial);
             e = hipGetLastError();
         }
         if (e == hipSuccess && P.nslots) {
-            k_sum_slots<<<dim3(P.nslots, nsets), 256, 0, st>>>(partial, (uint32_t)grid, wps, nparam, slots, P.map);
+            k_sum_slots<<<dim3(P.nslots, nsets), 256, 0, st>>>(partial, pg, pw, nparam, slots, P.map);
             e = hipGetLastError();
         }
     }

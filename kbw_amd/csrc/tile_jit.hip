@@ -398,12 +398,12 @@ std::string adj_pass_source(const AGateRec *g, uint32_t count, const std::string
          "    double *__restrict__ partial) {\n"
          "  __shared__ dv2 lds[TT * kChunk];\n";
     const uint32_t ns = std::max<uint32_t>(nslots, 1);
-    s += "  __shared__ double acc[" + std::to_string(ns) + " * TT];\n"
+    s += "  __shared__ double acc[" + std::to_string(ns) + " * kAccPerSlot];\n"
          "  __shared__ double ws[TT / 64];\n"
          "  const uint32_t tid = threadIdx.x;\n"
          "  const uint32_t set = blockIdx.x / wps, wk = blockIdx.x % wps;\n"
          "  gs += (size_t)set * " + std::to_string(count) + "u;\n";
-    s += "  for (uint32_t s = 0; s < " + std::to_string(ns) + "u; ++s) acc[s * TT + tid] = 0.0;\n"
+    s += "  adj_acc_zero(acc, " + std::to_string(ns) + "u, tid);\n"
          "  const uint64_t tmask = thread_offset(0, tid, ts);\n"
          "#pragma nounroll\n"
          "  for (uint32_t tl = wk; tl < tps; tl += wps) {\n"
@@ -428,7 +428,7 @@ std::string adj_pass_source(const AGateRec *g, uint32_t count, const std::string
         m += snprintf(line + m, sizeof line - m, "{ ");
         if (g[i].slot != kAdjNoSlot)
             m += snprintf(line + m, sizeof line - m,
-                          "acc[%uu * TT + tid] += adj_term_at<%uu, %uu, %uu>(v, lds, tidl, base, A.inv); ", g[i].slot,
+                          "adj_acc_add(acc, %uu, tid, adj_term_at<%uu, %uu, %uu>(v, lds, tidl, base, A.inv)); ", g[i].slot,
                           g[i].gkind, G.tpos, G.cm);
         snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tidl, A.inv); } }\n",
                  G.kind, G.tpos, G.cm, G.dlo, G.dhi);
@@ -441,7 +441,7 @@ std::string adj_pass_source(const AGateRec *g, uint32_t count, const std::string
          "    for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, ts)] = v[r];\n"
          "  }\n";
     s += "  for (uint32_t s = 0; s < " + std::to_string(nslots) + "u; ++s) {\n"
-         "    const double x = adj_block_sum(acc[s * TT + tid], ws);\n"
+         "    const double x = adj_block_sum(adj_acc_get(acc, s, tid), ws);\n"
          "    if (tid == 0) partial[(size_t)s * gridDim.x + blockIdx.x] = x;\n"
          "  }\n}\n";
     return s;

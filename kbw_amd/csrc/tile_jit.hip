@@ -419,8 +419,27 @@ std::string adj_pass_source(const AGateRec *g, uint32_t count, const std::string
          "#pragma unroll\n"
          "    for (int r = 0; r < TR; ++r) v[r] = a[slot_offset(dt, r, ts)];\n";
     char line[512];
+    // The pending per-thread phase tp of the forward passes (thread_phase_acc):
+    // a diagonal gate on thread bits and outside bits only is folded into tp,
+    // which multiplies phi and lambda at the same index alike.  Every term is
+    // a sum of conj(lambda_i) (G phi)_i, so a common factor of lambda_i and
+    // phi_i drops out (a unit factor, to rounding), and so does the factor of
+    // a partner phi across a bit tp does not depend on; tp is applied to the
+    // slots before a pair gate (and its term) on a thread bit in tp_dep, and
+    // before the store.  It restarts at 1 for every tile.
+    const uint32_t adj_tb = 8;  // thread bits of the reverse sweep's tile
+    auto diag_kind = [](uint32_t k) { return k == KZ || k == KP || k == KRZ; };
+    uint32_t tp_dep = 0;
+    bool tp_live = false;
+    s += "    dv2 tp = mk2(1.0, 0.0);\n";
     for (uint32_t i = 0; i < count; ++i) {
         const TGate &G = g[i].inv;
+        const bool diag = diag_kind(G.kind);
+        if (!diag && G.tpos < adj_tb && ((tp_dep >> G.tpos) & 1u)) {
+            s += "    thread_phase_apply(v, tp); tp = mk2(1.0, 0.0);\n    __builtin_amdgcn_sched_barrier(0);\n";
+            tp_dep = 0;
+        }
+        const bool fold = diag && G.dhi == 0 && (G.kind != KRZ || G.tpos == kOutside || G.tpos < adj_tb);
         int m = snprintf(line, sizeof line, "    { const AGate &A = gs[%u]; ", i);
         if (G.ocm)
             m += snprintf(line + m, sizeof line - m, "if ((base & %lluull) == %lluull) ", (unsigned long long)G.ocm,
@@ -430,13 +449,23 @@ std::string adj_pass_source(const AGateRec *g, uint32_t count, const std::string
             m += snprintf(line + m, sizeof line - m,
                           "adj_acc_add(acc, %uu, tid, adj_term_at<%uu, %uu, %uu>(v, lds, tidl, base, A.inv)); ", g[i].slot,
                           g[i].gkind, G.tpos, G.cm);
-        snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tidl, A.inv); } }\n",
-                 G.kind, G.tpos, G.cm, G.dlo, G.dhi);
+        if (fold) {
+            // the outside bits are tested by the branch around it
+            snprintf(line + m, sizeof line - m, "thread_phase_acc<%uu, %uu, %uu, false>(tp, base, tidl, A.inv); } }\n",
+                     G.kind, G.tpos, G.dlo);
+            tp_dep |= G.dlo;
+            if (G.kind == KRZ && G.tpos != kOutside) tp_dep |= 1u << G.tpos;
+            tp_live = true;
+        } else {
+            snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tidl, A.inv); } }\n",
+                     G.kind, G.tpos, G.cm, G.dlo, G.dhi);
+        }
         s += line;
         // one gate's term and update at a time: scheduled across gates, a
         // pass of 30-40 gates kept 230-300 registers live and spilled
         s += "    __builtin_amdgcn_sched_barrier(0);\n";
     }
+    if (tp_live) s += "    thread_phase_apply(v, tp);\n";
     s += "#pragma unroll\n"
          "    for (int r = 0; r < TR; ++r) a[slot_offset(dt, r, ts)] = v[r];\n"
          "  }\n";

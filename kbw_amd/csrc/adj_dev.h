@@ -12,7 +12,7 @@ namespace kbwh {
 namespace tile {
 namespace {
 
-constexpr uint32_t kAdjSlots = 24;  // parameterised gates per pass (LDS accumulators of TT doubles each)
+constexpr uint32_t kAdjSlots = 24;  // parameterised gates per pass (LDS accumulators of TT / 4 doubles each)
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 struct AGate {

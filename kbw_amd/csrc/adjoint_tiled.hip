@@ -36,6 +36,10 @@
 #define KBWH_TILE_BITS 12
 #define KBWH_TILE_TBITS 8
 #define KBWH_TILE_GATE_RBITS 3
+// contracted (products fused into the following adds), as the compiled
+// reverse kernels are: the gradient equals the program-order adjoint to
+// rounding either way
+#define KBWH_TILE_CONTRACT 1
 #include "tile_core.h"
 #include "adj_dev.h"
 

@@ -304,7 +304,9 @@ std::string sweep_source(const SweepKey *k, uint32_t count, const std::string &n
     for (uint32_t i = 0; i < count; ++i) wave = wave || (k[i].xt & 0xC0u) != 0;
     std::string s;
     s.reserve(2048 + count * 160);
-    s += "#include \"tile_dev.h\"\nusing namespace kbwh::tile;\n";
+    // contracted (w's products fused into its adds): equal to the generic
+    // sweep to rounding
+    s += "#define KBWH_TILE_CONTRACT 1\n#include \"tile_dev.h\"\nusing namespace kbwh::tile;\n";
     s += "extern \"C\" __global__ __launch_bounds__(TT) void " + name +
          "(const dv2 *__restrict__ a, TileSpec ts, const SweepString *__restrict__ ss, uint32_t ntiles,\n"
          "    double *__restrict__ partial) {\n";

@@ -826,6 +826,24 @@ __device__ __forceinline__ void thread_phase_apply(dv2 (&v)[TR], dv2 tp) {
     }
 }
 
+// An uncontrolled RZ (or fused diagonal gate of its form) on register bit
+// TP - TB: the |0> phase multiplies every amplitude of the tile, so it joins
+// the pending phase tp, and only the |1> slots take the ratio ph1 conj(ph0)
+// (the phases have unit modulus): 8 + 2 complex products instead of 16.
+template <uint32_t TP>
+__device__ __forceinline__ void rz_reg_fold(dv2 (&v)[TR], dv2 &tp, const TGate &G) {
+    static_assert(TP >= (uint32_t)TB && TP != kOutside, "register target");
+    constexpr uint32_t TM = 1u << (TP - TB);
+    const dv2 ph0 = mk2(G.ph0x, G.ph0y);
+    const dv2 ratio = cmul(mk2(G.ph1x, G.ph1y), mk2(ph0.x, -ph0.y));
+    tp = cmul(tp, ph0);
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+        slot_group(r);
+        if ((uint32_t)r & TM) v[r] = cmul(v[r], ratio);
+    }
+}
+
 template <uint32_t KIND, uint32_t TP, uint32_t DLO, bool OUT>
 __device__ __forceinline__ void thread_phase_acc(dv2 &tp, uint64_t base, uint32_t tid, const TGate &G) {
     const bool ok = (tid & DLO) == DLO && (!OUT || (base & G.ocm) == G.ocm);

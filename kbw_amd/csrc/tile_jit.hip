@@ -248,6 +248,14 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
             tp_thread = tp_any = true;
             continue;
         }
+        if (G.kind == KRZ && tp != kOutside && tp >= (uint32_t)kTb && cm == 0 && dhi == 0 && G.ocm == 0) {
+            // uncontrolled RZ on a register bit: its |0> phase joins tp (uniform over the tile)
+            snprintf(line, sizeof line, "  rz_reg_fold<%uu>(v, tp, gp[%u]);\n  __builtin_amdgcn_sched_barrier(0);\n", tp,
+                     i);
+            s += line;
+            tp_any = true;
+            continue;
+        }
         if (G.kind == KZ && G.ocm) {
             snprintf(line, sizeof line,
                      "  { uint32_t tg = tid; asm volatile(\"\" : \"+v\"(tg)); "
@@ -445,6 +453,9 @@ std::string adj_pass_source(const AGateRec *g, uint32_t count, const std::string
             tp_dep = 0;
         }
         const bool fold = diag && G.dhi == 0 && (G.kind != KRZ || G.tpos == kOutside || G.tpos < adj_tb);
+        // uncontrolled RZ on a register bit: its |0> phase joins tp (rz_reg_fold)
+        const bool rz_reg = G.kind == KRZ && G.tpos != kOutside && G.tpos >= adj_tb && G.cm == 0 && G.dhi == 0 &&
+                            G.ocm == 0;
         int m = snprintf(line, sizeof line, "    { const AGate &A = gs[%u]; ", i);
         if (G.ocm)
             m += snprintf(line + m, sizeof line - m, "if ((base & %lluull) == %lluull) ", (unsigned long long)G.ocm,
@@ -454,7 +465,10 @@ std::string adj_pass_source(const AGateRec *g, uint32_t count, const std::string
             m += snprintf(line + m, sizeof line - m,
                           "adj_acc_add(acc, %uu, tid, adj_term_at<%uu, %uu, %uu>(v, lds, tidl, base, A.inv)); ", g[i].slot,
                           g[i].gkind, G.tpos, G.cm);
-        if (fold) {
+        if (rz_reg) {
+            snprintf(line + m, sizeof line - m, "rz_reg_fold<%uu>(v, tp, A.inv); } }\n", G.tpos);
+            tp_live = true;
+        } else if (fold) {
             // the outside bits are tested by the branch around it
             snprintf(line + m, sizeof line - m, "thread_phase_acc<%uu, %uu, %uu, false>(tp, base, tidl, A.inv); } }\n",
                      G.kind, G.tpos, G.dlo);

@@ -159,7 +159,12 @@ __device__ __forceinline__ dv2 pair_out(dv2 m, dv2 o, const PairCoef &k) {
 // picks its partner's form at run time.  Controlled gates other than X use
 // the generic form (PK_GEN).  Every path is chosen by uniform two-way tests on
 // the gate record, never by a multi-way branch (see apply_pair_t).
-enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2, PK_Y = 3, PK_H = 4, PK_RX = 5 };
+enum { PK_X = 0, PK_TWO = 1, PK_GEN = 2, PK_Y = 3, PK_H = 4, PK_RX = 5, PK_HU = 6 };
+// An uncontrolled H in a compiled pass may leave out its factor 1/sqrt2
+// (PK_HU: m + o and o - m), which the pass then folds into its pending
+// phase: a factor common to every amplitude of the tile commutes with every
+// gate.  gate_at takes it as this pseudo-kind.
+constexpr uint32_t KH_UNSCALED = 8;
 
 // Control test for the amplitude in register slot r of thread tid, whose tile
 // index is tid + TT r: the thread bits are tested once per gate (tok) and the
@@ -183,6 +188,8 @@ __device__ __forceinline__ dv2 pair_val(dv2 m, dv2 o, bool one, const TGate &G) 
         const double sx = one ? o.x - m.x : m.x + o.x;
         const double sy = one ? o.y - m.y : m.y + o.y;
         return mk2(sx * G.d, sy * G.d);
+    } else if constexpr (PK == PK_HU) {
+        return mk2(one ? o.x - m.x : m.x + o.x, one ? o.y - m.y : m.y + o.y);
     } else if constexpr (PK == PK_TWO) {
         // RY's record has as = bu = 0 (set_coefficients): A = au + as sigma is
         // au and B = bu + bs sigma is sigma bs, exactly
@@ -206,6 +213,8 @@ __device__ __forceinline__ dv2 pair_val_lane(dv2 m, dv2 o, bool one, uint32_t sg
         return mk2(flip_sign_d(o.y, sg), flip_sign_d(o.x, sg ^ 0x80000000u));
     } else if constexpr (PK == PK_H) {
         return mk2((flip_sign_d(m.x, sg) + o.x) * G.d, (flip_sign_d(m.y, sg) + o.y) * G.d);
+    } else if constexpr (PK == PK_HU) {
+        return mk2(flip_sign_d(m.x, sg) + o.x, flip_sign_d(m.y, sg) + o.y);
     } else if constexpr (PK == PK_TWO) {
         const double B = flip_sign_d(G.bs, sg ^ 0x80000000u);
         return mk2(G.au * m.x + B * o.x, G.au * m.y + B * o.y);
@@ -890,6 +899,8 @@ __device__ __forceinline__ void gate_at(dv2 (&v)[TR], dv2 *lds, uint64_t base, u
         pair_at<TP, PK_Y, false, CM>(v, lds, tid, G);
     } else if constexpr (KIND == KH) {
         pair_at<TP, PK_H, false, CM>(v, lds, tid, G);
+    } else if constexpr (KIND == KH_UNSCALED) {
+        pair_at<TP, PK_HU, false, CM>(v, lds, tid, G);
     } else if constexpr (KIND == KRX) {
         pair_at<TP, PK_RX, false, CM>(v, lds, tid, G);
     } else {

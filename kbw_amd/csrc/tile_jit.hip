@@ -178,8 +178,10 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
     // tp is applied to the slots and restarts at 1; what is left is applied
     // before the store.  tp_thread: a thread-level gate was folded since the
     // last application (tp may differ between lanes); tp_any: anything was.
+    // uncontrolled H: run unscaled (PK_HU), its 1/sqrt2 joining tp
+    auto h_unscaled = [](const TGate &G) { return G.kind == KH && G.cm == 0 && G.ocm == 0; };
     bool any_diag = false;
-    for (uint32_t i = 0; i < count; ++i) any_diag = any_diag || is_diag(g[i]);
+    for (uint32_t i = 0; i < count; ++i) any_diag = any_diag || is_diag(g[i]) || h_unscaled(g[i]);
     if (any_diag) s += "  dv2 tp = mk2(1.0, 0.0);\n";
     uint32_t tp_dep = 0;
     bool tp_thread = false, tp_any = false;
@@ -274,8 +276,15 @@ std::string pass_source(const TGate *g, uint32_t count, uint32_t flags, const st
         if (G.ocm)
             m += snprintf(line + m, sizeof line - m, "if ((base & %lluull) == %lluull) ", (unsigned long long)G.ocm,
                           (unsigned long long)G.ocm);
-        snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tg, G); }\n", G.kind,
-                 tp, cm, dlo, dhi);
+        if (h_unscaled(G)) {
+            snprintf(line + m, sizeof line - m,
+                     "gate_at<KH_UNSCALED, %uu, %uu, %uu, %uu>(v, lds, base, tg, G); tp = tp * G.d; }\n", tp, cm, dlo,
+                     dhi);
+            tp_any = true;
+        } else {
+            snprintf(line + m, sizeof line - m, "gate_at<%uu, %uu, %uu, %uu, %uu>(v, lds, base, tg, G); }\n", G.kind,
+                     tp, cm, dlo, dhi);
+        }
         s += line;
         // one gate at a time: scheduled across gates, the temporaries of
         // several gates are live at once
